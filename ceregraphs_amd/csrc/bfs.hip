@@ -1,0 +1,651 @@
+// Single-root direction-optimising BFS for gfx950.
+//
+// Replaces the reference's on-device round loop f_spmv_iter
+// (bfs/implementation/src/bool_pe.csl:673-1161).  One round = one BFS level:
+//   * push round (compute_topdown, bool_pe.csl:865-887): the level-d queue is
+//     expanded over out-neighbours; a vertex is claimed by atomicOr on the
+//     visited bitmap (replacing the row OR-reduce, collectives_2d reduce_or),
+//     and every frontier predecessor of a vertex new at this level offers
+//     itself with atomicMin, so the parent is the lowest-id predecessor;
+//   * pull round (compute_bottomup, bool_pe.csl:899-921): every unvisited
+//     vertex scans its ascending in-neighbours and stops at the first one in
+//     the level-d frontier bitmap -- the first hit is the lowest id;
+//   * the decide step (reduce_done + the termination relay + the switch,
+//     bool_pe.csl:947-1156) reads device counters and picks the next
+//     direction or ends the search, with no host round trip.
+// The frontier lives in HBM both as a queue (push) and as a bitmap (pull).
+// Both are produced by every round, so either direction can follow either.
+//
+// Buffers are rotated by level: queue Q[d&1], bitmap F[d%3].  Round d reads
+// Q[d&1]/F[d%3], writes Q[(d+1)&1]/F[(d+1)%3] and zeroes F[(d+2)%3] (the
+// level d-1 bitmap, no longer read), so every bitmap is clear one round
+// before it is written.
+#include <cstring>
+#include <vector>
+
+#include "cg_common.h"
+
+namespace cg {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kHistCap = 4096;        // rounds recorded in the history arrays
+constexpr int64_t kBigDegree = 8192;  // push: whole-grid expansion at/above this
+constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this
+constexpr int64_t kPullWaveDegree = 128;  // pull: wavefront scan at/above this
+constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
+
+struct Ctrl {
+  unsigned long long qtail[2];  // queue lengths, Q[0] and Q[1]
+  unsigned long long big_tail;  // push: count of whole-grid vertices
+  unsigned long long nf;        // vertices discovered this round
+  unsigned long long mf;        // out-degree sum of the discovered vertices
+  unsigned long long in_sum;    // in-degree sum of the discovered vertices
+  unsigned long long scanned;   // adjacency entries read this round
+  long long mu;                 // in-edges of still-unvisited vertices
+  long long prev_nf;
+  long long tau;
+  long long max_levels;
+  long long rounds;
+  double alpha, beta;
+  int dir;  // direction of the current round: 0 push, 1 pull
+  int mode;
+  int done;
+  int status;
+  unsigned long long t0;
+  unsigned long long t_end;
+};
+
+struct Hist {
+  long long nf[kHistCap];
+  long long scanned[kHistCap];
+  unsigned long long t[kHistCap];
+  int dir[kHistCap];
+};
+
+struct Workspace {
+  int64_t n = 0;
+  int64_t nwords = 0;  // u32 words per bitmap, rounded up to 64 vertices
+  uint32_t *visited = nullptr;
+  uint32_t *fb[3] = {nullptr, nullptr, nullptr};
+  int32_t *q[2] = {nullptr, nullptr};
+  int32_t *bigq = nullptr;
+  int32_t *levels = nullptr;   // internal outputs (host-output calls)
+  uint32_t *parents = nullptr;
+  Ctrl *ctrl = nullptr;
+  Hist *hist = nullptr;
+  unsigned long long *scalars = nullptr;  // m / visited reductions
+};
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int ensure_ws1(DeviceGraph *g, Workspace *w) {
+  const int64_t n = g->n;
+  const int64_t nwords = ((n + 63) / 64) * 2;
+  size_t bm = align_up(sizeof(uint32_t) * nwords);
+  size_t qb = align_up(sizeof(int32_t) * n);
+  size_t total = bm * 4 + qb * 3 + align_up(sizeof(int32_t) * n) + align_up(sizeof(uint32_t) * n) +
+                 align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64);
+  if (g->ws1_bytes < total) {
+    if (g->ws1) hipFree(g->ws1);
+    g->ws1 = nullptr;
+    g->ws1_bytes = 0;
+    CG_HIP(hipMalloc(&g->ws1, total));
+    g->ws1_bytes = total;
+  }
+  char *p = (char *)g->ws1;
+  w->n = n;
+  w->nwords = nwords;
+  w->visited = (uint32_t *)p; p += bm;
+  for (int i = 0; i < 3; i++) { w->fb[i] = (uint32_t *)p; p += bm; }
+  for (int i = 0; i < 2; i++) { w->q[i] = (int32_t *)p; p += qb; }
+  w->bigq = (int32_t *)p; p += qb;
+  w->levels = (int32_t *)p; p += align_up(sizeof(int32_t) * n);
+  w->parents = (uint32_t *)p; p += align_up(sizeof(uint32_t) * n);
+  w->ctrl = (Ctrl *)p; p += align_up(sizeof(Ctrl));
+  w->hist = (Hist *)p; p += align_up(sizeof(Hist));
+  w->scalars = (unsigned long long *)p;
+  return CGBFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Block-level helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v,
+                                                         unsigned long long *lds) {
+  v = wave_sum_u64(v);
+  int w = threadIdx.x / kWave;
+  if (lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  unsigned long long s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x / kWave); i++) s += lds[i];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+struct LevelArgs {
+  const int64_t *out_ptr;
+  const int32_t *out_idx;
+  const int64_t *in_ptr;
+  const int32_t *in_idx;
+  int32_t *levels;
+  uint32_t *parents;
+  uint32_t *visited;
+  uint32_t *fb[3];
+  int32_t *q[2];
+  int32_t *bigq;
+  Ctrl *ctrl;
+  Hist *hist;
+  int64_t n;
+  int64_t nwords;
+};
+
+// ---------------------------------------------------------------------------
+// Init
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) bfs_init_fill(LevelArgs A) {
+  int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = tid; v < A.n; v += stride) {
+    A.levels[v] = -1;
+    A.parents[v] = kParentNone;
+  }
+  for (int64_t w = tid; w < A.nwords; w += stride) {
+    A.visited[w] = 0;
+    A.fb[0][w] = 0;
+    A.fb[1][w] = 0;
+    A.fb[2][w] = 0;
+  }
+}
+
+__global__ void bfs_init_source(LevelArgs A, int64_t s, int mode, double alpha, double beta,
+                                long long tau, long long max_levels, long long total_in) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  A.levels[s] = 0;
+  A.parents[s] = (uint32_t)s;
+  A.visited[s >> 5] |= 1u << (s & 31);
+  A.fb[0][s >> 5] |= 1u << (s & 31);
+  A.q[0][0] = (int32_t)s;
+  Ctrl *c = A.ctrl;
+  c->qtail[0] = 1;
+  c->qtail[1] = 0;
+  c->big_tail = 0;
+  c->nf = c->mf = c->in_sum = c->scanned = 0;
+  c->mu = total_in - (A.in_ptr[s + 1] - A.in_ptr[s]);
+  c->prev_nf = 1;
+  c->tau = tau;
+  c->max_levels = max_levels;
+  c->rounds = 0;
+  c->alpha = alpha;
+  c->beta = beta;
+  c->mode = mode;
+  c->dir = (mode == CGBFS_DIR_BOTTOMUP) ? 1 : 0;  // round 0 is push (bool_pe.csl:714-719)
+  c->done = 0;
+  c->status = 0;
+  c->t0 = wall_clock64();
+  c->t_end = c->t0;
+}
+
+// ---------------------------------------------------------------------------
+// Push round
+// ---------------------------------------------------------------------------
+struct PushLocal {
+  int32_t *lq;                // LDS queue
+  unsigned *lq_n;             // LDS count
+  unsigned long long mf, in_sum, scanned;
+};
+
+// Offers frontier vertex u as predecessor of v in round d.
+__device__ __forceinline__ void push_visit(const LevelArgs &A, int d, uint32_t u, int32_t v,
+                                           uint32_t *fnext, PushLocal &L,
+                                           unsigned long long *gtail, int32_t *gq) {
+  const uint32_t w = (uint32_t)v >> 5, b = 1u << (v & 31);
+  uint32_t vw = __builtin_nontemporal_load(&A.visited[w]);
+  if (!(vw & b)) {
+    uint32_t old = atomicOr(&A.visited[w], b);
+    if (!(old & b)) {
+      A.levels[v] = d + 1;
+      atomicOr(&fnext[w], b);
+      unsigned slot = atomicAdd(L.lq_n, 1u);
+      if (slot < (unsigned)kLdsQueue) L.lq[slot] = v;
+      else gq[atomicAdd(gtail, 1ull)] = v;
+      L.mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
+      L.in_sum += (unsigned long long)(A.in_ptr[v + 1] - A.in_ptr[v]);
+    }
+    atomicMin(&A.parents[v], u);
+  } else {
+    int lv = __hip_atomic_load(&A.levels[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lv == d + 1 || lv == -1) atomicMin(&A.parents[v], u);
+  }
+}
+
+__device__ __forceinline__ void push_flush(PushLocal &L, unsigned long long *gtail, int32_t *gq,
+                                           unsigned *lds_base) {
+  __syncthreads();
+  unsigned cnt = *L.lq_n;
+  if (cnt > (unsigned)kLdsQueue) cnt = kLdsQueue;
+  if (threadIdx.x == 0) *lds_base = (unsigned)(cnt ? atomicAdd(gtail, (unsigned long long)cnt) : 0);
+  __syncthreads();
+  unsigned long long base = *lds_base;
+  for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x) gq[base + i] = L.lq[i];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
+  Ctrl *c = A.ctrl;
+  if (c->done || c->dir != 0) return;
+  __shared__ int32_t lq[kLdsQueue];
+  __shared__ unsigned lq_n;
+  __shared__ unsigned lds_base;
+  __shared__ unsigned long long red[kWavesPerBlock];
+  if (threadIdx.x == 0) lq_n = 0;
+  __syncthreads();
+  PushLocal L{lq, &lq_n, 0, 0, 0};
+  const int cur = d & 1, nxt = (d + 1) & 1;
+  const int32_t *Q = A.q[cur];
+  int32_t *QN = A.q[nxt];
+  unsigned long long *gtail = &c->qtail[nxt];
+  uint32_t *fnext = A.fb[(d + 1) % 3];
+  uint32_t *fclear = A.fb[(d + 2) % 3];
+  const int64_t nq = (int64_t)c->qtail[cur];
+
+  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t w = gtid; w < A.nwords; w += gstride) fclear[w] = 0;
+
+  const int64_t wave = gtid / kWave;
+  const int64_t nwaves = gstride / kWave;
+  const int lane = lane_id();
+  for (int64_t base = wave * kWave; base < nq; base += nwaves * kWave) {
+    int64_t i = base + lane;
+    bool valid = i < nq;
+    int32_t u = valid ? Q[i] : 0;
+    int64_t beg = valid ? A.out_ptr[u] : 0;
+    int64_t end = valid ? A.out_ptr[u + 1] : 0;
+    int64_t deg = end - beg;
+    bool big = valid && deg >= kBigDegree;
+    bool mid = valid && !big && deg >= kWaveDegree;
+    bool small = valid && deg < kWaveDegree;
+    int64_t bslot = wave_append(&c->big_tail, big);
+    if (big) A.bigq[bslot] = u;
+    if (small) {
+      for (int64_t e = beg; e < end; e++) push_visit(A, d, (uint32_t)u, A.out_idx[e], fnext, L, gtail, QN);
+      L.scanned += (unsigned long long)deg;
+    }
+    if (mid) L.scanned += (unsigned long long)deg;
+    uint64_t mm = ballot64(mid);
+    while (mm) {
+      int j = __ffsll((unsigned long long)mm) - 1;
+      mm &= mm - 1;
+      int32_t uj = __shfl(u, j);
+      int64_t bj = __shfl(beg, j), ej = __shfl(end, j);
+      for (int64_t e = bj + lane; e < ej; e += kWave) push_visit(A, d, (uint32_t)uj, A.out_idx[e], fnext, L, gtail, QN);
+    }
+  }
+  push_flush(L, gtail, QN, &lds_base);
+  unsigned long long s;
+  s = block_sum(L.mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  s = block_sum(L.in_sum, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
+  s = block_sum(L.scanned, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+}
+
+// Whole-grid expansion of the push round's highest-degree vertices: every
+// thread of the grid takes a strided slice of each listed vertex's edges.
+__global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
+  Ctrl *c = A.ctrl;
+  if (c->done || c->dir != 0) return;
+  const int64_t nb = (int64_t)c->big_tail;
+  if (nb == 0) return;
+  __shared__ int32_t lq[kLdsQueue];
+  __shared__ unsigned lq_n;
+  __shared__ unsigned lds_base;
+  __shared__ unsigned long long red[kWavesPerBlock];
+  if (threadIdx.x == 0) lq_n = 0;
+  __syncthreads();
+  PushLocal L{lq, &lq_n, 0, 0, 0};
+  const int nxt = (d + 1) & 1;
+  int32_t *QN = A.q[nxt];
+  unsigned long long *gtail = &c->qtail[nxt];
+  uint32_t *fnext = A.fb[(d + 1) % 3];
+  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = 0; j < nb; j++) {
+    int32_t u = A.bigq[j];
+    int64_t beg = A.out_ptr[u], end = A.out_ptr[u + 1];
+    for (int64_t e = beg + gtid; e < end; e += gstride) push_visit(A, d, (uint32_t)u, A.out_idx[e], fnext, L, gtail, QN);
+    if (gtid == 0) L.scanned += (unsigned long long)(end - beg);
+  }
+  push_flush(L, gtail, QN, &lds_base);
+  unsigned long long s;
+  s = block_sum(L.mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  s = block_sum(L.in_sum, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
+  s = block_sum(L.scanned, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+}
+
+// ---------------------------------------------------------------------------
+// Pull round
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
+  Ctrl *c = A.ctrl;
+  if (c->done || c->dir != 1) return;
+  __shared__ unsigned long long red[kWavesPerBlock];
+  const uint32_t *fcur = A.fb[d % 3];
+  uint32_t *fnext = A.fb[(d + 1) % 3];
+  uint32_t *fclear = A.fb[(d + 2) % 3];
+  int32_t *QN = A.q[(d + 1) & 1];
+  unsigned long long *gtail = &c->qtail[(d + 1) & 1];
+  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t w = gtid; w < A.nwords; w += gstride) fclear[w] = 0;
+
+  const int lane = lane_id();
+  const int64_t wave = gtid / kWave;
+  const int64_t nwaves = gstride / kWave;
+  const int64_t nchunks = A.nwords / 2;
+  unsigned long long nf = 0, mf = 0, in_sum = 0, scanned = 0;
+  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
+    const int64_t w0 = ch * 2;
+    uint32_t vlo = A.visited[w0], vhi = A.visited[w0 + 1];
+    const int64_t v = ch * kWave + lane;
+    uint32_t myword = (lane < 32) ? vlo : vhi;
+    bool todo = v < A.n && !((myword >> (lane & 31)) & 1u);
+    if (!__any(todo)) continue;
+    int64_t beg = todo ? A.in_ptr[v] : 0, end = todo ? A.in_ptr[v + 1] : 0;
+    bool longrow = todo && (end - beg) >= kPullWaveDegree;
+    bool hit = false;
+    int32_t par = -1;
+    if (todo && !longrow) {
+      for (int64_t e = beg; e < end; e++) {
+        int32_t u = A.in_idx[e];
+        scanned++;
+        if ((fcur[u >> 5] >> (u & 31)) & 1u) {
+          hit = true;
+          par = u;
+          break;
+        }
+      }
+    }
+    // Long in-neighbour lists: the wave scans 64 entries at a time; the
+    // lowest lane with a frontier hit holds the lowest id of the batch.
+    uint64_t lr = ballot64(longrow);
+    while (lr) {
+      int j = __ffsll((unsigned long long)lr) - 1;
+      lr &= lr - 1;
+      int64_t bj = __shfl(beg, j), ej = __shfl(end, j);
+      int32_t found = -1;
+      for (int64_t e0 = bj; e0 < ej; e0 += kWave) {
+        int64_t e = e0 + lane;
+        bool h = false;
+        int32_t u = 0;
+        if (e < ej) {
+          u = A.in_idx[e];
+          h = (fcur[u >> 5] >> (u & 31)) & 1u;
+        }
+        uint64_t hm = ballot64(h);
+        if (hm) {
+          int k = __ffsll((unsigned long long)hm) - 1;
+          found = __shfl(u, k);
+          if (lane == 0) scanned += (unsigned long long)(e0 - bj + k + 1);
+          break;
+        }
+        if (lane == 0) scanned += (unsigned long long)((ej - e0) < kWave ? (ej - e0) : kWave);
+      }
+      if (lane == j && found >= 0) {
+        hit = true;
+        par = found;
+      }
+    }
+    if (hit) {
+      A.parents[v] = (uint32_t)par;
+      A.levels[v] = d + 1;
+      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
+      in_sum += (unsigned long long)(end - beg);
+    }
+    uint64_t hm = ballot64(hit);
+    if (hm) {
+      if (lane == 0) {
+        uint32_t lo = (uint32_t)hm, hi = (uint32_t)(hm >> 32);
+        if (lo) { fnext[w0] = lo; A.visited[w0] = vlo | lo; }
+        if (hi) { fnext[w0 + 1] = hi; A.visited[w0 + 1] = vhi | hi; }
+        nf += (unsigned long long)__popcll(hm);
+      }
+      int64_t slot = wave_append(gtail, hit);
+      if (hit) QN[slot] = (int32_t)v;
+    }
+  }
+  unsigned long long s;
+  s = block_sum(nf, red);
+  (void)s;
+  s = block_sum(mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  s = block_sum(in_sum, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
+  s = block_sum(scanned, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+}
+
+// ---------------------------------------------------------------------------
+// Decide: termination, direction, history (one thread)
+// ---------------------------------------------------------------------------
+__global__ void bfs_decide(LevelArgs A, int d, int64_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctrl *c = A.ctrl;
+  if (c->done) return;
+  const int nxt = (d + 1) & 1;
+  const long long nf = (long long)c->qtail[nxt];
+  unsigned long long now = wall_clock64();
+  if (d < kHistCap) {
+    A.hist->nf[d] = nf;
+    A.hist->scanned[d] = (long long)c->scanned;
+    A.hist->dir[d] = c->dir;
+    A.hist->t[d] = now;
+  }
+  c->rounds = d + 1;
+  c->t_end = now;
+  c->qtail[d & 1] = 0;
+  c->big_tail = 0;
+  c->mu -= (long long)c->in_sum;
+  const long long mf = (long long)c->mf;
+  c->nf = c->mf = c->in_sum = c->scanned = 0;
+  if (nf == 0) {
+    c->done = 1;
+    return;
+  }
+  int dir = c->dir;
+  switch (c->mode) {
+    case CGBFS_DIR_TOPDOWN: dir = 0; break;
+    case CGBFS_DIR_BOTTOMUP: dir = 1; break;
+    case CGBFS_DIR_TAU:
+      if (dir == 0 && nf > c->tau) dir = 1;  // one-way, bool_pe.csl:1078-1088
+      break;
+    default:  // Beamer et al. alpha/beta
+      if (dir == 0) {
+        if ((double)mf > (double)c->mu / c->alpha) dir = 1;
+      } else {
+        if ((double)nf < (double)n / c->beta && nf < c->prev_nf) dir = 0;
+      }
+  }
+  c->prev_nf = nf;
+  c->dir = dir;
+  if (d + 1 >= c->max_levels) {
+    c->status = CGBFS_ETIMEOUT;
+    c->done = 1;
+  }
+}
+
+// m (bfs_timing.py:390-395) and the visited count, reduced over levels >= 0.
+__global__ void __launch_bounds__(kBlock) bfs_count(const int32_t *levels, const int32_t *w,
+                                                    int64_t n, unsigned long long *out) {
+  __shared__ unsigned long long red[kWavesPerBlock];
+  unsigned long long m = 0, cnt = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (levels[v] >= 0) {
+      m += (unsigned long long)w[v];
+      cnt++;
+    }
+  }
+  unsigned long long s = block_sum(m, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&out[0], s);
+  s = block_sum(cnt, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&out[1], s);
+}
+
+__global__ void zero_u64(unsigned long long *p, int k) {
+  if (threadIdx.x < (unsigned)k) p[threadIdx.x] = 0;
+}
+
+int grid_blocks(int device, int mult) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+    cus = 256;
+  return cus * (mult > 0 ? mult : 8);
+}
+
+double wall_clock_hz(int device) {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+    khz = 100000;
+  return (double)khz * 1000.0;
+}
+
+void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o) {
+  memset(o, 0, sizeof(*o));
+  if (opt) *o = *opt;
+  if (o->alpha <= 0) o->alpha = 14.0;
+  if (o->beta <= 0) o->beta = 24.0;
+  if (o->tau <= 0) o->tau = (int64_t)(0.15 * (double)n + 0.5);
+  if (o->max_levels <= 0) o->max_levels = n + 1;
+  if (o->grid_mult <= 0) o->grid_mult = 8;
+}
+
+}  // namespace cg
+
+using namespace cg;
+
+extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32_t *levels,
+                         int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats) {
+  if (!h) return set_error(CGBFS_EINVAL, "graph is null");
+  DeviceGraph *g = &h->g;
+  if (source < 0 || source >= g->n) return set_error(CGBFS_EINVAL, "source out of range");
+  if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
+  cgbfs_options o;
+  fill_options(opt, g->n, &o);
+  if (o.direction < 0 || o.direction > 3) return set_error(CGBFS_EINVAL, "unknown direction mode");
+  CG_HIP(hipSetDevice(g->device));
+  Workspace ws;
+  int rc = ensure_ws1(g, &ws);
+  if (rc) return rc;
+
+  LevelArgs A;
+  A.out_ptr = g->out_ptr;
+  A.out_idx = g->out_idx;
+  A.in_ptr = g->in_ptr;
+  A.in_idx = g->in_idx;
+  A.levels = on_device ? levels : ws.levels;
+  A.parents = on_device ? (uint32_t *)parents : ws.parents;
+  A.visited = ws.visited;
+  for (int i = 0; i < 3; i++) A.fb[i] = ws.fb[i];
+  A.q[0] = ws.q[0];
+  A.q[1] = ws.q[1];
+  A.bigq = ws.bigq;
+  A.ctrl = ws.ctrl;
+  A.hist = ws.hist;
+  A.n = g->n;
+  A.nwords = ws.nwords;
+
+  hipStream_t st = g->stream;
+  const int blocks = grid_blocks(g->device, o.grid_mult);
+  hipEvent_t e0, e1;
+  CG_HIP(hipEventCreate(&e0));
+  CG_HIP(hipEventCreate(&e1));
+  CG_HIP(hipEventRecord(e0, st));
+  bfs_init_fill<<<blocks, kBlock, 0, st>>>(A);
+  bfs_init_source<<<1, 64, 0, st>>>(A, source, o.direction, o.alpha, o.beta, (long long)o.tau,
+                                    (long long)o.max_levels, (long long)g->total_out_edges);
+  CG_HIP(hipGetLastError());
+
+  // Rounds are enqueued in chunks without host synchronisation; kernels of
+  // rounds past the end see ctrl->done and return at once.  The host polls
+  // the done flag once per chunk; e1 is re-recorded after every chunk so it
+  // ends up right after the chunk in which the search finished.
+  int64_t d = 0;
+  int chunk = o.rounds_hint > 0 ? o.rounds_hint : 12;
+  Ctrl hc;
+  for (;;) {
+    for (int k = 0; k < chunk; k++, d++) {
+      bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
+      bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);
+      bfs_pull_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
+      bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n);
+    }
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipEventRecord(e1, st));
+    CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    CG_HIP(hipStreamSynchronize(st));
+    if (hc.done) break;
+    if (d > o.max_levels + chunk) break;
+    chunk = 4;
+  }
+  float ms = 0;
+  CG_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (hc.status) return set_error(hc.status, "BFS exceeded max_levels");
+
+  double out_ms = 0;
+  if (!on_device) {
+    hipEvent_t o0, o1;
+    CG_HIP(hipEventCreate(&o0));
+    CG_HIP(hipEventCreate(&o1));
+    CG_HIP(hipEventRecord(o0, st));
+    CG_HIP(hipMemcpyAsync(parents, A.parents, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
+    CG_HIP(hipMemcpyAsync(levels, A.levels, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
+    CG_HIP(hipEventRecord(o1, st));
+    CG_HIP(hipEventSynchronize(o1));
+    float oms = 0;
+    hipEventElapsedTime(&oms, o0, o1);
+    out_ms = oms;
+    hipEventDestroy(o0);
+    hipEventDestroy(o1);
+  }
+
+  if (stats) {
+    const double hz = wall_clock_hz(g->device);
+    stats->rounds_completed = hc.rounds;
+    stats->total_ms = ms;
+    stats->output_ms = out_ms;
+    int64_t nh = hc.rounds < kHistCap ? hc.rounds : kHistCap;
+    if (stats->max_hist < nh) nh = stats->max_hist;
+    if (nh > 0 && (stats->nf_history || stats->direction_history || stats->edges_scanned || stats->level_ms)) {
+      std::vector<char> hb(sizeof(Hist));
+      CG_HIP(hipMemcpy(hb.data(), A.hist, sizeof(Hist), hipMemcpyDeviceToHost));
+      const Hist *H = (const Hist *)hb.data();
+      for (int64_t r = 0; r < nh; r++) {
+        if (stats->nf_history) stats->nf_history[r] = H->nf[r];
+        if (stats->direction_history) stats->direction_history[r] = H->dir[r];
+        if (stats->edges_scanned) stats->edges_scanned[r] = H->scanned[r];
+        if (stats->level_ms) {
+          unsigned long long prev = r == 0 ? hc.t0 : H->t[r - 1];
+          stats->level_ms[r] = (double)(H->t[r] - prev) * 1e3 / hz;
+        }
+      }
+    }
+    unsigned long long sc[2] = {0, 0};
+    zero_u64<<<1, 64, 0, st>>>(ws.scalars, 2);
+    bfs_count<<<grid_blocks(g->device, 2), kBlock, 0, st>>>(A.levels, g->m_weight, g->n, ws.scalars);
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipMemcpyAsync(sc, ws.scalars, sizeof(sc), hipMemcpyDeviceToHost, st));
+    CG_HIP(hipStreamSynchronize(st));
+    stats->m_edges = (int64_t)sc[0];
+    stats->visited_count = (int64_t)sc[1];
+  }
+  return CGBFS_OK;
+}

@@ -1,0 +1,150 @@
+/*
+ * cgbfs.h -- C ABI of the MI355X (gfx950) BFS library libcgbfs.so.
+ *
+ * This boundary replaces the reference's device/plugin interface for the BFS
+ * path: the compiled CSL program behind SdkRuntime, i.e. the compile-time
+ * parameters of bfs/implementation/src/layout_bool.csl:18-38, the exported
+ * device buffers of layout_bool.csl:84-116 and the entry points
+ * f_spmv_iter / f_enable_tsc / f_tic / f_toc / f_sync_hostdevice
+ * (layout_bool.csl:118-122) that bfs/scripts/run_bfs.py:288-504 and
+ * run_graph500.py:246-375 drive with memcpy_h2d / launch / memcpy_d2h.
+ *
+ * Plain C types only: pointers, sizes, status codes.  No exceptions cross the
+ * ABI.  Every call returns CGBFS_OK (0) or a negative CGBFS_E* code, and
+ * cgbfs_last_error() returns a message for the calling thread's last failure.
+ *
+ * Matrix orientation (kept from bfs/implementation/graph_loader.py:76-90): a
+ * nonzero M[r][c] is the edge c -> r.  "out" arrays are column-compressed M
+ * (out-neighbours of each vertex, the push view); "in" arrays are
+ * row-compressed M (in-neighbours, ascending, the pull view).
+ *
+ * Threading: calls on one graph handle are serialised by the caller and run
+ * on that handle's own HIP stream; distinct handles may be used from
+ * distinct threads.
+ */
+#ifndef CGBFS_H
+#define CGBFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGBFS_API __attribute__((visibility("default")))
+
+#define CGBFS_OK 0
+#define CGBFS_EINVAL (-1)   /* bad argument (range, size, null pointer)   */
+#define CGBFS_ENOMEM (-2)   /* device or pinned-host allocation failed    */
+#define CGBFS_EHIP (-3)     /* a HIP runtime call failed                   */
+#define CGBFS_ETIMEOUT (-4) /* level cap reached before the search ended  */
+#define CGBFS_ENODEV (-5)   /* no usable gfx950 device                     */
+
+/* Direction policies for the level loop. */
+#define CGBFS_DIR_BEAMER 0   /* alpha/beta, both ways (default)                  */
+#define CGBFS_DIR_TOPDOWN 1  /* push every level                                 */
+#define CGBFS_DIR_BOTTOMUP 2 /* pull every level                                 */
+#define CGBFS_DIR_TAU 3      /* reference rule: one-way push->pull once nf > tau
+                                (bool_pe.csl:1078-1088, run_bfs.py:239-243)      */
+
+typedef struct cgbfs_graph cgbfs_graph;
+
+/* Per-search options.  Zero-initialised fields take the defaults noted. */
+typedef struct {
+  int32_t direction;  /* CGBFS_DIR_*                                  (0)    */
+  int32_t rounds_hint; /* rounds enqueued before the first host poll (12) */
+  double alpha;       /* push->pull when m_f > m_u / alpha            (14)   */
+  double beta;        /* pull->push when n_f < n / beta               (24)   */
+  int64_t tau;        /* CGBFS_DIR_TAU threshold on nf                (0.15n)*/
+  int32_t block;      /* threads per block for level kernels          (256)  */
+  int32_t grid_mult;  /* blocks per CU for grid-stride kernels        (8)    */
+  int64_t max_levels; /* abort with CGBFS_ETIMEOUT beyond this        (n+1)  */
+} cgbfs_options;
+
+/* Per-search statistics.  The caller owns the history arrays (capacity
+ * max_hist entries each; any of them may be NULL).  Per-round meaning
+ * follows bool_pe.csl: nf_history[r] = vertices discovered in round r
+ * (:1063-1065); direction_history[r] = 0 push / 1 pull for round r
+ * (:1090-1097); rounds_completed = max_level + 1 (:989,1068). */
+typedef struct {
+  int64_t rounds_completed;
+  int64_t max_hist;
+  int64_t *nf_history;
+  int32_t *direction_history;
+  int64_t *edges_scanned;   /* adjacency entries read in round r          */
+  double *level_ms;         /* device time of round r (device wall clock) */
+  double total_ms;          /* search time on the graph's stream (HIP events):
+                               init -> parents/levels resident in HBM      */
+  double output_ms;         /* device->host output copy (host outputs)     */
+  int64_t visited_count;    /* single-root only                            */
+  int64_t m_edges;          /* GTEPS m for a single root (bfs_timing.py:390-395) */
+} cgbfs_stats;
+
+/* --- library ------------------------------------------------------------ */
+CGBFS_API const char *cgbfs_last_error(void);
+CGBFS_API int cgbfs_version(void);
+CGBFS_API int cgbfs_device_count(int32_t *count);
+
+/* --- graph lifecycle (Kernel 1: construction, once per graph) ----------- */
+
+/* Copies host CSR/CSC arrays into HBM.  For symmetric != 0 the in_* arrays
+ * may be NULL (the out view is shared).  Indices must be sorted ascending
+ * within each in-row (pull first-hit = lowest id).  Replaces preprocess()
+ * (bfs/implementation/preprocess_bool.py:29-229) and the matrix upload of
+ * run_bfs.py:338-358. */
+CGBFS_API int cgbfs_graph_create(int32_t device, int64_t n, int64_t nnz, const int64_t *out_ptr,
+                       const int32_t *out_idx, const int64_t *in_ptr, const int32_t *in_idx,
+                       int32_t symmetric, cgbfs_graph **out);
+
+/* Builds an R-MAT graph on the GPU, edge-identical to
+ * datasets/gen_rmat.py:15-44 for the same numpy PCG64 stream.  pcg[0..3] =
+ * state_hi, state_lo, inc_hi, inc_lo of numpy.random.default_rng(seed)'s
+ * bit generator.  a, ab, abc are the cumulative quadrant thresholds A, A+B,
+ * A+B+C as float64.  symmetric != 0: symmetrise + drop self-loops + dedup
+ * (gen_rmat.py:34-44).  symmetric == 0: directed edge u->v stored as
+ * M[v][u], self-loops dropped, duplicates merged. */
+CGBFS_API int cgbfs_graph_create_rmat(int32_t device, int32_t scale, int64_t edgefactor,
+                            const uint64_t *pcg, double a, double ab, double abc,
+                            int32_t symmetric, cgbfs_graph **out);
+
+CGBFS_API int cgbfs_graph_info(const cgbfs_graph *g, int64_t *n, int64_t *nnz, int32_t *symmetric);
+
+/* Copies the device adjacency back (any pointer may be NULL to skip). */
+CGBFS_API int cgbfs_graph_download(const cgbfs_graph *g, int64_t *out_ptr, int32_t *out_idx,
+                         int64_t *in_ptr, int32_t *in_idx);
+
+/* Out-degree >= 1 excluding self-loops is the reference's root rule
+ * (run_graph500.py:119-120); this returns out-degree per vertex. */
+CGBFS_API int cgbfs_graph_out_degree(const cgbfs_graph *g, int64_t *deg);
+
+CGBFS_API int cgbfs_graph_destroy(cgbfs_graph *g);
+
+/* --- search (Kernel 2) ----------------------------------------------------*/
+
+/* Single-root direction-optimising BFS.  parents/levels: int32[n], caller
+ * allocated; on_device != 0 means they are device pointers on the graph's
+ * device, else host memory.  Unreached = -1, parents[source] = source,
+ * parents[v] = lowest-id in-neighbour one level closer.  Replaces
+ * runner.launch("f_spmv_iter") plus the parent readback
+ * (run_bfs.py:381-409, device_io.py:160-205). */
+CGBFS_API int cgbfs_bfs(cgbfs_graph *g, int64_t source, int32_t *parents, int32_t *levels,
+              int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats);
+
+/* Multi-source BFS: nroots <= 64 roots packed as the bits of one uint64 word
+ * per vertex.  parents: int32[nroots][n]; levels: int16[nroots][n]; lane l
+ * is exactly cgbfs_bfs(roots[l]) (levels cast to int16; a search deeper
+ * than 32767 levels fails with CGBFS_EINVAL).  m_lanes (may be NULL):
+ * int64[nroots], GTEPS m per lane.  stats->nf_history counts vertices with
+ * at least one newly reached lane per round. */
+CGBFS_API int cgbfs_msbfs(cgbfs_graph *g, const int64_t *roots, int32_t nroots, int32_t *parents,
+                int16_t *levels, int32_t on_device, const cgbfs_options *opt,
+                cgbfs_stats *stats, int64_t *m_lanes);
+
+/* Blocks until all work queued on the graph's stream has finished. */
+CGBFS_API int cgbfs_sync(cgbfs_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGBFS_H */

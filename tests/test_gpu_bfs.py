@@ -1,0 +1,189 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bar: bit-exact levels and parents (lowest-id rule), identical per-round
+discovery counts and round counts, for every direction policy; MS-BFS lanes
+identical to single-root searches.  Full-size graphs (s21, s24) are checked
+on sampled roots against the oracle and through size-independent properties
+(lane == single-root, structural validation on sampled lanes)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import golden
+from oracle import oracle as O
+from ceregraphs_amd import gen_rmat
+from ceregraphs_amd.engine import DeviceGraph
+
+pytestmark = pytest.mark.gpu
+
+DIRECTIONS = ["beamer", "topdown", "bottomup", "tau"]
+
+
+def _device(g):
+  return DeviceGraph.from_csr(g.n, g.out_ptr, g.out_idx, g.in_ptr, g.in_idx, g.symmetric)
+
+
+def _edges_graph(edges, n, directed=True):
+  src = np.array([e[0] for e in edges], dtype=np.int64)
+  dst = np.array([e[1] for e in edges], dtype=np.int64)
+  if not directed:
+    src, dst = np.concatenate([src, dst]), np.concatenate([dst, src])
+  M = sp.coo_matrix((np.ones(len(src)), (dst, src)), shape=(n, n))
+  return O.Graph.from_matrix(M)
+
+
+def _check_single(dg, g, s, direction, **kw):
+  r = dg.bfs(s, direction=direction, **kw)
+  lev, par, hist, rounds = O.bfs(g, s)
+  np.testing.assert_array_equal(r.levels, lev, err_msg=f"levels src={s} dir={direction}")
+  np.testing.assert_array_equal(r.parents, par, err_msg=f"parents src={s} dir={direction}")
+  assert r.stats.rounds_completed == rounds
+  np.testing.assert_array_equal(r.stats.nf_history, hist)
+  assert r.stats.m_edges == O.m_edges(g, lev)
+  assert r.stats.visited_count == int((lev >= 0).sum())
+  return r
+
+
+def test_c1_plumbing(c1_graph):
+  """Configuration 1 (s10, 4x4-balanced, source 0) against the reference's
+  own scipy result and the lowest-id rule."""
+  s = golden("c1_scipy_src0.npz")
+  with _device(c1_graph) as dg:
+    for d in DIRECTIONS:
+      r = _check_single(dg, c1_graph, 0, d)
+      np.testing.assert_array_equal(r.levels, s["levels"])
+      assert O.validate(c1_graph, 0, r.parents, r.levels) == []
+    r = dg.bfs(0, direction="tau")
+    # nf = 471 > tau = round(0.15 * 1024) after round 0 (bool_pe.csl:1078-1088)
+    assert list(r.stats.direction_history) == [0, 1, 1, 1]
+
+
+@pytest.mark.parametrize("scale,ef,seed,directed", [
+    (10, 16, 0, False), (12, 8, 3, False), (14, 16, 1, False), (13, 8, 10, True),
+    (15, 16, 5, True)])
+def test_gpu_generator_matches_numpy(scale, ef, seed, directed):
+  abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+  with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
+    assert dg.n == n and dg.nnz == len(ii) and dg.symmetric == (not directed)
+    gip, gii, gop, goi = dg.download()
+    np.testing.assert_array_equal(gip, ip)
+    np.testing.assert_array_equal(gii, ii)
+    np.testing.assert_array_equal(gop, op)
+    np.testing.assert_array_equal(goi, oi)
+
+
+@pytest.mark.parametrize("scale,ef,seed,directed", [
+    (12, 16, 1, False), (14, 8, 2, False), (12, 8, 10, True), (14, 32, 15, True)])
+def test_single_root_parity_all_directions(scale, ef, seed, directed):
+  abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+  g = O.Graph(n, ip, ii, op, oi, not directed)
+  deg = np.diff(g.out_ptr)
+  roots = [int(np.argmax(deg)), int(np.nonzero(deg)[0][0]), int(np.nonzero(deg == 0)[0][0]),
+           *np.random.default_rng(seed).choice(np.nonzero(deg)[0], 3, replace=False).tolist()]
+  with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
+    for s in roots:
+      for d in DIRECTIONS:
+        _check_single(dg, g, s, d)
+      _check_single(dg, g, s, "beamer", alpha=2.0, beta=100.0)
+
+
+def test_edge_cases():
+  """Self-loop at the source, two components, isolated source, a deep path
+  (many rounds: exercises the chunked round loop), a star whose hub has more
+  than 8192 neighbours (whole-grid push), n not a multiple of 64."""
+  cases = []
+  cases.append((_edges_graph([(0, 0), (0, 1), (1, 2), (3, 4)], 5), [0, 3, 2]))
+  n = 3000
+  cases.append((_edges_graph([(i, i + 1) for i in range(n - 1)], n, directed=False), [0, 1500, n - 1]))
+  cases.append((_edges_graph([(0, i) for i in range(1, 20001)] + [(20000, 20001)], 20003,
+                             directed=False), [0, 7, 20001, 20002]))
+  cases.append((_edges_graph([(i, (i * 7 + 3) % 777) for i in range(777)], 777), [0, 5, 776]))
+  for g, roots in cases:
+    with _device(g) as dg:
+      for s in roots:
+        for d in DIRECTIONS:
+          r = _check_single(dg, g, s, d)
+          assert O.validate(g, s, r.parents, r.levels) == []
+
+
+def test_device_outputs_and_repeat_determinism():
+  import torch
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(13, 16, 4)
+  g = O.Graph(n, ip, ii, op, oi, True)
+  with DeviceGraph.from_rmat(13, 16, 4) as dg:
+    par = torch.empty(n, dtype=torch.int32, device="cuda")
+    lev = torch.empty(n, dtype=torch.int32, device="cuda")
+    lev0, par0, _, _ = O.bfs(g, 9)
+    for _ in range(3):
+      dg.bfs(9, device_out=(par, lev))
+      np.testing.assert_array_equal(lev.cpu().numpy(), lev0)
+      np.testing.assert_array_equal(par.cpu().numpy(), par0)
+
+
+@pytest.mark.parametrize("scale,ef,seed,directed,nroots", [
+    (12, 16, 1, False, 64), (13, 8, 2, False, 7), (12, 16, 12, True, 64), (11, 8, 3, False, 1)])
+def test_msbfs_lanes_equal_oracle(scale, ef, seed, directed, nroots):
+  abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+  g = O.Graph(n, ip, ii, op, oi, not directed)
+  deg = np.diff(op)
+  rng = np.random.default_rng(seed + 100)
+  roots = rng.choice(n, nroots, replace=False)
+  roots[0] = int(np.argmax(deg))
+  if nroots > 2:
+    roots[1] = roots[2]  # a repeated root
+  with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
+    for d in DIRECTIONS:
+      r = dg.msbfs(roots, direction=d)
+      for l, s in enumerate(roots):
+        lev, par, _, rounds = O.bfs(g, int(s))
+        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16), err_msg=f"lane {l} dir {d}")
+        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d}")
+        assert r.m_lanes[l] == O.m_edges(g, lev)
+
+
+def test_c2_scale21_sampled_roots():
+  """Configuration 2 graph (s21 ef16 seed 1), direction-optimising searches
+  from sampled roots, bit-exact against the oracle."""
+  with DeviceGraph.from_rmat(21, 16, 1) as dg:
+    ip, ii, op, oi = dg.download()
+    g = O.Graph(dg.n, ip, ii, op, oi, True)
+    deg = np.diff(op)
+    roots = np.random.default_rng(2).choice(np.nonzero(deg)[0], 3, replace=False)
+    for s in roots:
+      _check_single(dg, g, int(s), "beamer")
+    _check_single(dg, g, int(roots[0]), "tau")
+    r = dg.msbfs(roots)
+    for l, s in enumerate(roots):
+      single = dg.bfs(int(s))
+      np.testing.assert_array_equal(r.levels[l], single.levels.astype(np.int16))
+      np.testing.assert_array_equal(r.parents[l], single.parents)
+
+
+def test_c3_scale24_msbfs_lanes_equal_single_root():
+  """Configuration 3 graph (s24 ef16 seed 3), 64 roots from seed 4 in one
+  multi-source pass, each lane bit-exact to the single-root search; two
+  lanes checked against the oracle in full."""
+  import torch
+  with DeviceGraph.from_rmat(24, 16, 3) as dg:
+    n = dg.n
+    deg = dg.out_degree()
+    roots = np.random.default_rng(4).choice(np.nonzero(deg > 0)[0], 64, replace=False)
+    par = torch.empty(64 * n, dtype=torch.int32, device="cuda")
+    lev = torch.empty(64 * n, dtype=torch.int16, device="cuda")
+    r = dg.msbfs(roots, device_out=(par, lev))
+    par2 = torch.empty(n, dtype=torch.int32, device="cuda")
+    lev2 = torch.empty(n, dtype=torch.int32, device="cuda")
+    for l in range(64):
+      single = dg.bfs(int(roots[l]), device_out=(par2, lev2))
+      assert torch.equal(lev[l * n:(l + 1) * n], lev2.to(torch.int16)), f"lane {l}"
+      assert torch.equal(par[l * n:(l + 1) * n], par2), f"lane {l}"
+      assert r.m_lanes[l] == single.stats.m_edges
+    ip, ii, op, oi = dg.download()
+    g = O.Graph(n, ip, ii, op, oi, True)
+    for l in (0, 63):
+      lev0, par0, _, _ = O.bfs(g, int(roots[l]))
+      np.testing.assert_array_equal(lev[l * n:(l + 1) * n].cpu().numpy(), lev0.astype(np.int16))
+      np.testing.assert_array_equal(par[l * n:(l + 1) * n].cpu().numpy(), par0)
