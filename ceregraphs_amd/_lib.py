@@ -58,6 +58,8 @@ class Stats(ctypes.Structure):
       ("nf_history", ctypes.POINTER(ctypes.c_int64)),
       ("direction_history", ctypes.POINTER(ctypes.c_int32)),
       ("edges_scanned", ctypes.POINTER(ctypes.c_int64)),
+      ("rows_scanned", ctypes.POINTER(ctypes.c_int64)),
+      ("pairs_found", ctypes.POINTER(ctypes.c_int64)),
       ("level_ms", ctypes.POINTER(ctypes.c_double)),
       ("total_ms", ctypes.c_double),
       ("output_ms", ctypes.c_double),
@@ -105,6 +107,14 @@ def load():
       return _lib
     if not os.path.exists(LIB_PATH):
       raise ImportError(f"{LIB_PATH} is missing: build it with `python -m ceregraphs_amd.build`")
+    # PyTorch ships its own HIP runtime whose soname matches the one this
+    # library links (libamdhip64.so.7).  Loading torch first makes the
+    # dynamic linker bind this library to that same runtime, so a process
+    # that also uses torch tensors has exactly one HIP runtime.
+    try:
+      import torch  # noqa: F401  pylint: disable=unused-import,import-outside-toplevel
+    except ImportError:
+      pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
       fn = getattr(lib, name)

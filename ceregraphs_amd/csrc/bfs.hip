@@ -42,6 +42,7 @@ struct Ctrl {
   unsigned long long mf;        // out-degree sum of the discovered vertices
   unsigned long long in_sum;    // in-degree sum of the discovered vertices
   unsigned long long scanned;   // adjacency entries read this round
+  unsigned long long rows;      // adjacency lists opened this round
   long long mu;                 // in-edges of still-unvisited vertices
   long long prev_nf;
   long long tau;
@@ -59,6 +60,7 @@ struct Ctrl {
 struct Hist {
   long long nf[kHistCap];
   long long scanned[kHistCap];
+  long long rows[kHistCap];
   unsigned long long t[kHistCap];
   int dir[kHistCap];
 };
@@ -171,7 +173,7 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, int mode, double alpha, 
   c->qtail[0] = 1;
   c->qtail[1] = 0;
   c->big_tail = 0;
-  c->nf = c->mf = c->in_sum = c->scanned = 0;
+  c->nf = c->mf = c->in_sum = c->scanned = c->rows = 0;
   c->mu = total_in - (A.in_ptr[s + 1] - A.in_ptr[s]);
   c->prev_nf = 1;
   c->tau = tau;
@@ -193,7 +195,7 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, int mode, double alpha, 
 struct PushLocal {
   int32_t *lq;                // LDS queue
   unsigned *lq_n;             // LDS count
-  unsigned long long mf, in_sum, scanned;
+  unsigned long long mf, in_sum, scanned, rows;
 };
 
 // Offers frontier vertex u as predecessor of v in round d.
@@ -241,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
   __shared__ unsigned long long red[kWavesPerBlock];
   if (threadIdx.x == 0) lq_n = 0;
   __syncthreads();
-  PushLocal L{lq, &lq_n, 0, 0, 0};
+  PushLocal L{lq, &lq_n, 0, 0, 0, 0};
   const int cur = d & 1, nxt = (d + 1) & 1;
   const int32_t *Q = A.q[cur];
   int32_t *QN = A.q[nxt];
@@ -274,6 +276,7 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
       L.scanned += (unsigned long long)deg;
     }
     if (mid) L.scanned += (unsigned long long)deg;
+    if (valid) L.rows++;
     uint64_t mm = ballot64(mid);
     while (mm) {
       int j = __ffsll((unsigned long long)mm) - 1;
@@ -291,6 +294,8 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
   if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
   s = block_sum(L.scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  s = block_sum(L.rows, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
 // Whole-grid expansion of the push round's highest-degree vertices: every
@@ -306,7 +311,7 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
   __shared__ unsigned long long red[kWavesPerBlock];
   if (threadIdx.x == 0) lq_n = 0;
   __syncthreads();
-  PushLocal L{lq, &lq_n, 0, 0, 0};
+  PushLocal L{lq, &lq_n, 0, 0, 0, 0};
   const int nxt = (d + 1) & 1;
   int32_t *QN = A.q[nxt];
   unsigned long long *gtail = &c->qtail[nxt];
@@ -327,6 +332,8 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
   if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
   s = block_sum(L.scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  s = block_sum(L.rows, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -349,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
   const int64_t wave = gtid / kWave;
   const int64_t nwaves = gstride / kWave;
   const int64_t nchunks = A.nwords / 2;
-  unsigned long long nf = 0, mf = 0, in_sum = 0, scanned = 0;
+  unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0;
   for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
     const int64_t w0 = ch * 2;
     uint32_t vlo = A.visited[w0], vhi = A.visited[w0 + 1];
@@ -361,6 +368,7 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
     bool longrow = todo && (end - beg) >= kPullWaveDegree;
     bool hit = false;
     int32_t par = -1;
+    if (todo) rows++;
     if (todo && !longrow) {
       for (int64_t e = beg; e < end; e++) {
         int32_t u = A.in_idx[e];
@@ -414,21 +422,20 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
         uint32_t lo = (uint32_t)hm, hi = (uint32_t)(hm >> 32);
         if (lo) { fnext[w0] = lo; A.visited[w0] = vlo | lo; }
         if (hi) { fnext[w0 + 1] = hi; A.visited[w0 + 1] = vhi | hi; }
-        nf += (unsigned long long)__popcll(hm);
       }
       int64_t slot = wave_append(gtail, hit);
       if (hit) QN[slot] = (int32_t)v;
     }
   }
   unsigned long long s;
-  s = block_sum(nf, red);
-  (void)s;
   s = block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
   s = block_sum(in_sum, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
   s = block_sum(scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  s = block_sum(rows, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,6 +451,7 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n) {
   if (d < kHistCap) {
     A.hist->nf[d] = nf;
     A.hist->scanned[d] = (long long)c->scanned;
+    A.hist->rows[d] = (long long)c->rows;
     A.hist->dir[d] = c->dir;
     A.hist->t[d] = now;
   }
@@ -453,7 +461,7 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n) {
   c->big_tail = 0;
   c->mu -= (long long)c->in_sum;
   const long long mf = (long long)c->mf;
-  c->nf = c->mf = c->in_sum = c->scanned = 0;
+  c->nf = c->mf = c->in_sum = c->scanned = c->rows = 0;
   if (nf == 0) {
     c->done = 1;
     return;
@@ -563,9 +571,9 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
 
   hipStream_t st = g->stream;
   const int blocks = grid_blocks(g->device, o.grid_mult);
-  hipEvent_t e0, e1;
-  CG_HIP(hipEventCreate(&e0));
-  CG_HIP(hipEventCreate(&e1));
+  EventPool pool;
+  hipEvent_t e0 = pool.make(), e1 = pool.make();
+  if (!e0 || !e1) return set_error(CGBFS_EHIP, "hipEventCreate failed");
   CG_HIP(hipEventRecord(e0, st));
   bfs_init_fill<<<blocks, kBlock, 0, st>>>(A);
   bfs_init_source<<<1, 64, 0, st>>>(A, source, o.direction, o.alpha, o.beta, (long long)o.tau,
@@ -579,11 +587,19 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   int64_t d = 0;
   int chunk = o.rounds_hint > 0 ? o.rounds_hint : 12;
   Ctrl hc;
+  std::vector<hipEvent_t> rev;  // two per round: around its traversal kernels
   for (;;) {
     for (int k = 0; k < chunk; k++, d++) {
+      hipEvent_t ra = nullptr, rb = nullptr;
+      if (d < kHistCap && (ra = pool.make()) && (rb = pool.make())) {
+        rev.push_back(ra);
+        rev.push_back(rb);
+        hipEventRecord(ra, st);
+      }
       bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
       bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);
       bfs_pull_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
+      if (rb) hipEventRecord(rb, st);
       bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n);
     }
     CG_HIP(hipGetLastError());
@@ -596,15 +612,12 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   }
   float ms = 0;
   CG_HIP(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   if (hc.status) return set_error(hc.status, "BFS exceeded max_levels");
 
   double out_ms = 0;
   if (!on_device) {
-    hipEvent_t o0, o1;
-    CG_HIP(hipEventCreate(&o0));
-    CG_HIP(hipEventCreate(&o1));
+    hipEvent_t o0 = pool.make(), o1 = pool.make();
+    if (!o0 || !o1) return set_error(CGBFS_EHIP, "hipEventCreate failed");
     CG_HIP(hipEventRecord(o0, st));
     CG_HIP(hipMemcpyAsync(parents, A.parents, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
     CG_HIP(hipMemcpyAsync(levels, A.levels, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
@@ -613,18 +626,15 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
     float oms = 0;
     hipEventElapsedTime(&oms, o0, o1);
     out_ms = oms;
-    hipEventDestroy(o0);
-    hipEventDestroy(o1);
   }
 
   if (stats) {
-    const double hz = wall_clock_hz(g->device);
     stats->rounds_completed = hc.rounds;
     stats->total_ms = ms;
     stats->output_ms = out_ms;
     int64_t nh = hc.rounds < kHistCap ? hc.rounds : kHistCap;
     if (stats->max_hist < nh) nh = stats->max_hist;
-    if (nh > 0 && (stats->nf_history || stats->direction_history || stats->edges_scanned || stats->level_ms)) {
+    if (nh > 0) {
       std::vector<char> hb(sizeof(Hist));
       CG_HIP(hipMemcpy(hb.data(), A.hist, sizeof(Hist), hipMemcpyDeviceToHost));
       const Hist *H = (const Hist *)hb.data();
@@ -632,9 +642,12 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
         if (stats->nf_history) stats->nf_history[r] = H->nf[r];
         if (stats->direction_history) stats->direction_history[r] = H->dir[r];
         if (stats->edges_scanned) stats->edges_scanned[r] = H->scanned[r];
-        if (stats->level_ms) {
-          unsigned long long prev = r == 0 ? hc.t0 : H->t[r - 1];
-          stats->level_ms[r] = (double)(H->t[r] - prev) * 1e3 / hz;
+        if (stats->rows_scanned) stats->rows_scanned[r] = H->rows[r];
+        if (stats->pairs_found) stats->pairs_found[r] = H->nf[r];
+        if (stats->level_ms && 2 * r + 1 < (int64_t)rev.size()) {
+          float rm = 0;
+          hipEventElapsedTime(&rm, rev[2 * r], rev[2 * r + 1]);
+          stats->level_ms[r] = rm;
         }
       }
     }

@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/cgbfs.h"
 
@@ -58,11 +59,29 @@ struct DeviceGraph {
   // Lazily sized multi-source workspace.
   void *wsm = nullptr;
   size_t wsm_bytes = 0;
+  // Per-graph multi-source data (segment lists of high-degree vertices),
+  // built on first use; freed through msdata_free.
+  void *msdata = nullptr;
+  void (*msdata_free)(void *) = nullptr;
   // Pinned host mailbox for the done flag polled between level chunks.
   int32_t *host_flag = nullptr;
 };
 
 int graph_finalize(DeviceGraph *g);  // computes m_weight, stream, mailbox
+
+// HIP events owned for the length of one call (destroyed on every exit path).
+struct EventPool {
+  std::vector<hipEvent_t> ev;
+  hipEvent_t make() {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ev.push_back(e);
+    return e;
+  }
+  ~EventPool() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Device helpers

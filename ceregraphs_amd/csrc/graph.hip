@@ -273,6 +273,7 @@ static void graph_free(DeviceGraph *g) {
   if (g->m_weight) hipFree(g->m_weight);
   if (g->ws1) hipFree(g->ws1);
   if (g->wsm) hipFree(g->wsm);
+  if (g->msdata && g->msdata_free) g->msdata_free(g->msdata);
   if (g->host_flag) hipHostFree(g->host_flag);
   if (g->stream) hipStreamDestroy(g->stream);
   // DeviceGraph is the first and only member of the handle.

@@ -4,23 +4,32 @@
 // Per vertex v the library keeps three lane words: vis[v] (lanes that have
 // reached v), and the frontier pair fr[v] / nx[v] (lanes for which v is in
 // the current / next level).  Every lane follows exactly the single-root
-// semantics of bfs.hip (and of bool_pe.csl's round loop): lane l's level is
-// its hop distance from roots[l] and its parent is the lowest-id
+// semantics of bfs.hip (and of bool_pe.csl's round loop, :865-1156): lane
+// l's level is its hop distance from roots[l] and its parent is the lowest-id
 // in-neighbour one level closer.
 //
-// A round is one of:
-//   * push: active vertices (queue) OR their frontier word into nx[w] of
-//     every out-neighbour w, masked by the level-start vis[w]; the first
-//     thread to make nx[w] non-zero appends w to the next queue.  Parents are
-//     then resolved by a pull restricted to that queue (no per-lane atomics):
-//     each newly reached vertex scans its ascending in-neighbours and gives
-//     each new lane the first neighbour whose frontier word has that lane;
-//   * pull: every vertex with an unreached lane scans its ascending
-//     in-neighbours, ORing frontier words, and stops as soon as every
-//     unreached lane has been found.
+// Work tiers (RMAT degrees span 1 .. ~10^6):
+//   * degree <  64      one thread per vertex;
+//   * degree <  kSeg    one wavefront per vertex, 64 neighbours per step;
+//   * degree >= kSeg    the adjacency is cut into kSeg-entry segments listed
+//                       once per graph; each segment is one wavefront's work
+//                       item, and segments of one vertex combine per lane
+//                       with atomicMin / atomicOr in a small scratch area.
+//
+// Pull round (ms_pull + ms_pull_fin): every vertex with an unreached lane
+// ORs the frontier words of its ascending in-neighbours; within a step of
+// 64 neighbours an exclusive prefix-OR across the wavefront gives each lane
+// of each vertex its first (lowest-id) hit.  Stops once every needed lane is
+// found.
+// Push round (ms_push x2 + ms_prep): pass 1 ORs frontier words into nx[w]
+// of out-neighbours (masked by the level-start vis[w]); the first writer of
+// a vertex appends it to the next queue.  ms_prep then marks the new
+// (lane, vertex) pairs, and pass 2 offers every frontier predecessor with a
+// per-lane atomicMin, so the parent is again the lowest id.
 // Outputs parents[l][v] (int32) and levels[l][v] (int16) are written once
 // per (lane, vertex): at discovery, or by the final pass for unreached
 // pairs.  No initial fill of the 6 B x 64 x V output arrays is needed.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -31,15 +40,18 @@ namespace cg {
 constexpr int kMsBlock = 256;
 constexpr int kMsWaves = kMsBlock / kWave;
 constexpr int kMsHist = 4096;
+constexpr int64_t kSeg = 1024;      // segment length and wavefront-tier cap
+constexpr int64_t kThreadDeg = 64;  // thread tier below this degree
 
 int grid_blocks(int device, int mult);
-double wall_clock_hz(int device);
 void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o);
 
 struct MsCtrl {
   unsigned long long qtail[2];
   unsigned long long mf;       // out-degree sum of the next active set
   unsigned long long scanned;  // adjacency entries read this round
+  unsigned long long rows;     // adjacency lists opened this round
+  unsigned long long pairs;    // (lane, vertex) pairs reached this round
   long long prev_nf;
   long long tau;
   long long max_levels;
@@ -50,14 +62,25 @@ struct MsCtrl {
   int mode;
   int done;
   int status;
-  unsigned long long t0, t_end;
 };
 
 struct MsHist {
   long long nf[kMsHist];
   long long scanned[kMsHist];
-  unsigned long long t[kMsHist];
+  long long rows[kMsHist];
+  long long pairs[kMsHist];
   int dir[kMsHist];
+};
+
+// Static segment list of one adjacency view (per graph).
+struct SegList {
+  int64_t nseg = 0;
+  int64_t nbig = 0;
+  int32_t *seg_v = nullptr;    // vertex of each segment
+  int32_t *seg_bid = nullptr;  // big-vertex index of each segment
+  int64_t *seg_beg = nullptr;  // first adjacency entry of each segment
+  int32_t *big_v = nullptr;    // big vertices, by index
+  uint8_t *is_big = nullptr;   // per vertex
 };
 
 struct MsArgs {
@@ -72,12 +95,21 @@ struct MsArgs {
   int32_t *q[2];
   MsCtrl *ctrl;
   MsHist *hist;
+  // big-vertex segments, pull (in view) and push (out view)
+  SegList pin, pout;
+  uint32_t *cand;    // [pin.nbig][64] per-lane first-hit candidates
+  uint64_t *bgot;    // [pin.nbig] lanes found this round
   int64_t n;
   uint64_t act;      // mask of lanes in use
   int nroots;
 };
 
 static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct MsGraphData {
+  SegList pin, pout;
+  void *mem = nullptr;
+};
 
 struct MsWorkspace {
   uint64_t *vis, *fw[2];
@@ -86,13 +118,97 @@ struct MsWorkspace {
   MsHist *hist;
   unsigned long long *mlane;
   int64_t *roots;
+  uint32_t *cand;
+  uint64_t *bgot;
 };
 
-static int ensure_wsm(DeviceGraph *g, MsWorkspace *w) {
+// ---------------------------------------------------------------------------
+// Per-graph segment lists (built on the host from the offset arrays once).
+// ---------------------------------------------------------------------------
+static void seg_plan(const std::vector<int64_t> &ptr, int64_t n, std::vector<int32_t> &sv,
+                     std::vector<int32_t> &sb, std::vector<int64_t> &sg, std::vector<int32_t> &bv,
+                     std::vector<uint8_t> &isb) {
+  isb.assign(n, 0);
+  for (int64_t v = 0; v < n; v++) {
+    int64_t deg = ptr[v + 1] - ptr[v];
+    if (deg < kSeg) continue;
+    int32_t bid = (int32_t)bv.size();
+    bv.push_back((int32_t)v);
+    isb[v] = 1;
+    for (int64_t b = ptr[v]; b < ptr[v + 1]; b += kSeg) {
+      sv.push_back((int32_t)v);
+      sb.push_back(bid);
+      sg.push_back(b);
+    }
+  }
+}
+
+static int build_seglists(DeviceGraph *g, MsGraphData *md) {
   const int64_t n = g->n;
+  std::vector<int64_t> ptr_in(n + 1), ptr_out;
+  CG_HIP(hipMemcpy(ptr_in.data(), g->in_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+  std::vector<int32_t> sv[2], sb[2], bv[2];
+  std::vector<int64_t> sg[2];
+  std::vector<uint8_t> isb[2];
+  seg_plan(ptr_in, n, sv[0], sb[0], sg[0], bv[0], isb[0]);
+  const bool shared = g->in_ptr == g->out_ptr;
+  if (!shared) {
+    ptr_out.resize(n + 1);
+    CG_HIP(hipMemcpy(ptr_out.data(), g->out_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    seg_plan(ptr_out, n, sv[1], sb[1], sg[1], bv[1], isb[1]);
+  }
+  const int views = shared ? 1 : 2;
+  size_t total = 0;
+  for (int k = 0; k < views; k++)
+    total += al(sv[k].size() * 4 + 4) + al(sb[k].size() * 4 + 4) + al(sg[k].size() * 8 + 8) +
+             al(bv[k].size() * 4 + 4) + al((size_t)n);
+  CG_HIP(hipMalloc(&md->mem, total));
+  char *p = (char *)md->mem;
+  for (int k = 0; k < views; k++) {
+    SegList &L = (k == 0) ? md->pin : md->pout;
+    L.nseg = (int64_t)sv[k].size();
+    L.nbig = (int64_t)bv[k].size();
+    L.seg_v = (int32_t *)p; p += al(sv[k].size() * 4 + 4);
+    L.seg_bid = (int32_t *)p; p += al(sb[k].size() * 4 + 4);
+    L.seg_beg = (int64_t *)p; p += al(sg[k].size() * 8 + 8);
+    L.big_v = (int32_t *)p; p += al(bv[k].size() * 4 + 4);
+    L.is_big = (uint8_t *)p; p += al((size_t)n);
+    if (L.nseg) {
+      CG_HIP(hipMemcpy(L.seg_v, sv[k].data(), sv[k].size() * 4, hipMemcpyHostToDevice));
+      CG_HIP(hipMemcpy(L.seg_bid, sb[k].data(), sb[k].size() * 4, hipMemcpyHostToDevice));
+      CG_HIP(hipMemcpy(L.seg_beg, sg[k].data(), sg[k].size() * 8, hipMemcpyHostToDevice));
+      CG_HIP(hipMemcpy(L.big_v, bv[k].data(), bv[k].size() * 4, hipMemcpyHostToDevice));
+    }
+    CG_HIP(hipMemcpy(L.is_big, isb[k].data(), (size_t)n, hipMemcpyHostToDevice));
+  }
+  if (shared) md->pout = md->pin;
+  return CGBFS_OK;
+}
+
+static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
+  const int64_t n = g->n;
+  if (!g->msdata) {
+    MsGraphData *md = new MsGraphData();
+    int rc = build_seglists(g, md);
+    if (rc) {
+      if (md->mem) hipFree(md->mem);
+      delete md;
+      return rc;
+    }
+    g->msdata = md;
+    g->msdata_free = [](void *q) {
+      MsGraphData *m = (MsGraphData *)q;
+      if (m->mem) hipFree(m->mem);
+      delete m;
+    };
+  }
+  MsGraphData *md = (MsGraphData *)g->msdata;
+  *mdp = md;
+  const int64_t nbig = md->pin.nbig;
   size_t wb = al(sizeof(uint64_t) * n), qb = al(sizeof(int32_t) * n);
   size_t total = 3 * wb + 2 * qb + al(sizeof(MsCtrl)) + al(sizeof(MsHist)) +
-                 al(sizeof(unsigned long long) * 64) + al(sizeof(int64_t) * 64);
+                 al(sizeof(unsigned long long) * 64) + al(sizeof(int64_t) * 64) +
+                 al(sizeof(uint32_t) * 64 * (nbig + 1)) + al(sizeof(uint64_t) * (nbig + 1));
   if (g->wsm_bytes < total) {
     if (g->wsm) hipFree(g->wsm);
     g->wsm = nullptr;
@@ -109,8 +225,34 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w) {
   w->ctrl = (MsCtrl *)p; p += al(sizeof(MsCtrl));
   w->hist = (MsHist *)p; p += al(sizeof(MsHist));
   w->mlane = (unsigned long long *)p; p += al(sizeof(unsigned long long) * 64);
-  w->roots = (int64_t *)p;
+  w->roots = (int64_t *)p; p += al(sizeof(int64_t) * 64);
+  w->cand = (uint32_t *)p; p += al(sizeof(uint32_t) * 64 * (nbig + 1));
+  w->bgot = (uint64_t *)p;
   return CGBFS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t x, int off) {
+  uint32_t lo = __shfl_up((uint32_t)x, off), hi = __shfl_up((uint32_t)(x >> 32), off);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
+  uint32_t lo = __shfl((uint32_t)x, src), hi = __shfl((uint32_t)(x >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive prefix-OR across the wavefront.
+__device__ __forceinline__ uint64_t wave_scan_or(uint64_t x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint64_t y = shfl_up64(x, off);
+    if (lane >= off) x |= y;
+  }
+  return x;
 }
 
 __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
@@ -126,6 +268,61 @@ __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
   return s;
 }
 
+// Writes parents/levels for lanes `lanes` of vertex v whose parent is u.
+__device__ __forceinline__ void ms_emit(const MsArgs &A, uint64_t lanes, int32_t u, int64_t v,
+                                        int16_t lev) {
+  while (lanes) {
+    int l = __ffsll((unsigned long long)lanes) - 1;
+    lanes &= lanes - 1;
+    int64_t o = (int64_t)l * A.n + v;
+    A.parents[o] = u;
+    A.levels[o] = lev;
+  }
+}
+
+// Wavefront scan of in-neighbours [beg, end) of v for lanes `need`
+// (wave-uniform arguments).  Emits first hits when `emit`, else offers them
+// to cand[] with atomicMin.  Returns the lanes found (wave-uniform).
+__device__ __forceinline__ uint64_t ms_wave_pull(const MsArgs &A, const uint64_t *fr, int64_t v,
+                                                 int64_t beg, int64_t end, uint64_t need,
+                                                 int16_t lev, uint32_t *cand,
+                                                 unsigned long long &scanned) {
+  const int lane = lane_id();
+  uint64_t got = 0;
+  for (int64_t e0 = beg; e0 < end; e0 += kWave) {
+    int64_t e = e0 + lane;
+    uint64_t h = 0;
+    int32_t u = 0;
+    if (e < end) {
+      u = A.in_idx[e];
+      h = fr[u] & need & ~got;
+    }
+    uint64_t inc = wave_scan_or(h);
+    uint64_t before = shfl_up64(inc, 1);
+    if (lane == 0) before = 0;
+    uint64_t mine = h & ~before;
+    if (mine) {
+      if (cand) {
+        uint64_t b = mine;
+        while (b) {
+          int l = __ffsll((unsigned long long)b) - 1;
+          b &= b - 1;
+          atomicMin(&cand[l], (uint32_t)u);
+        }
+      } else {
+        ms_emit(A, mine, u, v, lev);
+      }
+    }
+    got |= shfl64(inc, kWave - 1);
+    if (lane == 0) scanned += (unsigned long long)((end - e0) < kWave ? (end - e0) : kWave);
+    if (got == need) break;
+  }
+  return got;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kMsBlock) ms_init_fill(MsArgs A) {
   int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -134,6 +331,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_init_fill(MsArgs A) {
     A.fw[0][v] = 0;
     A.fw[1][v] = 0;
   }
+  for (int64_t i = tid; i < A.pin.nbig * 64; i += stride) A.cand[i] = 0xFFFFFFFFu;
+  for (int64_t i = tid; i < A.pin.nbig; i += stride) A.bgot[i] = 0;
 }
 
 __global__ void ms_init_roots(MsArgs A, const int64_t *roots, int mode, double alpha, double beta,
@@ -155,8 +354,7 @@ __global__ void ms_init_roots(MsArgs A, const int64_t *roots, int mode, double a
   MsCtrl *c = A.ctrl;
   c->qtail[0] = nq;
   c->qtail[1] = 0;
-  c->mf = 0;
-  c->scanned = 0;
+  c->mf = c->scanned = c->rows = c->pairs = 0;
   c->prev_nf = (long long)nq;
   c->tau = tau;
   c->max_levels = max_levels;
@@ -171,46 +369,138 @@ __global__ void ms_init_roots(MsArgs A, const int64_t *roots, int mode, double a
   c->dir = dir;
   c->done = 0;
   c->status = 0;
-  c->t0 = wall_clock64();
-  c->t_end = c->t0;
 }
 
-// Writes parents/levels for lanes `lanes` of vertex v whose parent is u.
-__device__ __forceinline__ void ms_emit(const MsArgs &A, uint64_t lanes, int32_t u, int64_t v,
-                                        int16_t lev) {
-  while (lanes) {
-    int l = __ffsll((unsigned long long)lanes) - 1;
-    lanes &= lanes - 1;
-    int64_t o = (int64_t)l * A.n + v;
-    A.parents[o] = u;
-    A.levels[o] = lev;
-  }
-}
-
-// Scans v's ascending in-neighbours for the lanes in `need`; emits
-// (lane, first neighbour with that lane).  Returns the lanes found.
-__device__ __forceinline__ uint64_t ms_pull_vertex(const MsArgs &A, const uint64_t *fr, int64_t v,
-                                                   uint64_t need, int16_t lev,
-                                                   unsigned long long &scanned) {
-  uint64_t got = 0;
-  int64_t beg = A.in_ptr[v], end = A.in_ptr[v + 1];
-  for (int64_t e = beg; e < end; e++) {
-    int32_t u = A.in_idx[e];
-    uint64_t h = fr[u] & need & ~got;
-    scanned++;
-    if (h) {
-      ms_emit(A, h, u, v, lev);
-      got |= h;
-      if (got == need) break;
-    }
-  }
-  return got;
-}
-
-// Push: OR frontier words into the next words of out-neighbours.
-__global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d) {
+// Pull round over non-big vertices (thread and wavefront tiers), then the
+// big-vertex segments (one wavefront per segment, per-lane atomicMin into
+// cand[], lanes into bgot[]).
+__global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
   MsCtrl *c = A.ctrl;
-  if (c->done || c->dir != 0) return;
+  __shared__ unsigned long long red[kMsWaves];
+  const uint64_t *fr = A.fw[d & 1];
+  uint64_t *nx = A.fw[(d + 1) & 1];
+  int32_t *QN = A.q[(d + 1) & 1];
+  unsigned long long *tail = &c->qtail[(d + 1) & 1];
+  const int16_t lev = (int16_t)(d + 1);
+  unsigned long long scanned = 0, mf = 0, rows = 0, pairs = 0;
+  const int lane = lane_id();
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t wave = gtid / kWave, nwaves = stride / kWave;
+  // Chunks of 64 consecutive vertices per wavefront.
+  const int64_t nch = (A.n + kWave - 1) / kWave;
+  for (int64_t ch = wave; ch < nch; ch += nwaves) {
+    const int64_t v = ch * kWave + lane;
+    uint64_t vw = 0, need = 0;
+    int64_t beg = 0, end = 0;
+    if (v < A.n && !A.pin.is_big[v]) {
+      vw = A.vis[v];
+      need = A.act & ~vw;
+      if (need) {
+        beg = A.in_ptr[v];
+        end = A.in_ptr[v + 1];
+      }
+    }
+    const int64_t deg = end - beg;
+    uint64_t got = 0;
+    if (need && deg < kThreadDeg) {
+      rows++;
+      for (int64_t e = beg; e < end; e++) {
+        int32_t u = A.in_idx[e];
+        uint64_t h = fr[u] & need & ~got;
+        scanned++;
+        if (h) {
+          ms_emit(A, h, u, v, lev);
+          got |= h;
+          if (got == need) break;
+        }
+      }
+    }
+    uint64_t wt = ballot64(need && deg >= kThreadDeg);
+    while (wt) {
+      int j = __ffsll((unsigned long long)wt) - 1;
+      wt &= wt - 1;
+      int64_t vj = ch * kWave + j;
+      uint64_t nj = shfl64(need, j);
+      int64_t bj = __shfl(beg, j), ej = __shfl(end, j);
+      uint64_t gj = ms_wave_pull(A, fr, vj, bj, ej, nj, lev, nullptr, scanned);
+      if (lane == j) {
+        got = gj;
+        rows++;
+      }
+    }
+    bool any = got != 0;
+    if (any) {
+      nx[v] = got;
+      A.vis[v] = vw | got;
+      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
+      pairs += (unsigned long long)__popcll(got);
+    }
+    int64_t slot = wave_append(tail, any);
+    if (any) QN[slot] = (int32_t)v;
+  }
+  // Big vertices: one segment per wavefront.
+  for (int64_t s = wave; s < A.pin.nseg; s += nwaves) {
+    int32_t v = A.pin.seg_v[s];
+    uint64_t need = A.act & ~A.vis[v];
+    if (!need) continue;
+    int32_t bid = A.pin.seg_bid[s];
+    int64_t beg = A.pin.seg_beg[s];
+    int64_t rend = A.in_ptr[v + 1];
+    int64_t end = beg + kSeg < rend ? beg + kSeg : rend;
+    uint64_t gj = ms_wave_pull(A, fr, v, beg, end, need, lev, &A.cand[(int64_t)bid * 64], scanned);
+    if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
+  }
+  unsigned long long s = ms_block_sum(scanned, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  s = ms_block_sum(mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  s = ms_block_sum(rows, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
+  s = ms_block_sum(pairs, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->pairs, s);
+}
+
+// Big vertices after a pull round: turn per-lane candidates into outputs.
+__global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int d) {
+  MsCtrl *c = A.ctrl;
+  uint64_t *nx = A.fw[(d + 1) & 1];
+  int32_t *QN = A.q[(d + 1) & 1];
+  unsigned long long *tail = &c->qtail[(d + 1) & 1];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nround = ((A.pin.nbig + stride - 1) / stride) * stride;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nround; b += stride) {
+    uint64_t got = 0;
+    int32_t v = 0;
+    if (b < A.pin.nbig) {
+      v = A.pin.big_v[b];
+      got = A.bgot[b];
+      if (got) {
+        uint64_t lanes = got;
+        while (lanes) {
+          int l = __ffsll((unsigned long long)lanes) - 1;
+          lanes &= lanes - 1;
+          int64_t o = (int64_t)l * A.n + v;
+          A.parents[o] = (int32_t)A.cand[b * 64 + l];
+          A.levels[o] = (int16_t)(d + 1);
+          A.cand[b * 64 + l] = 0xFFFFFFFFu;
+        }
+        A.bgot[b] = 0;
+        nx[v] = got;
+        A.vis[v] |= got;
+        atomicAdd(&c->mf, (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]));
+        atomicAdd(&c->pairs, (unsigned long long)__popcll(got));
+      }
+    }
+    int64_t slot = wave_append(tail, got != 0);
+    if (got) QN[slot] = v;
+  }
+}
+
+// Push pass (phase 0: OR into nx and append first writers; phase 1: offer
+// predecessors to the new pairs with per-lane atomicMin).
+__global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) {
+  MsCtrl *c = A.ctrl;
   __shared__ unsigned long long red[kMsWaves];
   const uint64_t *fr = A.fw[d & 1];
   uint64_t *nx = A.fw[(d + 1) & 1];
@@ -221,97 +511,99 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d) {
   const int lane = lane_id();
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t wave = gtid / kWave, nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-  unsigned long long scanned = 0;
-  // One wavefront per active vertex: edges strided over the 64 lanes.
-  for (int64_t i = wave; i < nq; i += nwaves) {
-    int32_t u = Q[i];
+  unsigned long long scanned = 0, rows = 0;
+  // Work items: queue entries (skipping big vertices), then big segments of
+  // active big vertices.  Both loops are wavefront-uniform.
+  const int64_t nitems = nq + A.pout.nseg;
+  for (int64_t it = wave; it < nitems; it += nwaves) {
+    int32_t u;
+    int64_t beg, end;
+    if (it < nq) {
+      u = Q[it];
+      if (A.pout.is_big[u]) continue;
+      beg = A.out_ptr[u];
+      end = A.out_ptr[u + 1];
+    } else {
+      int64_t s = it - nq;
+      u = A.pout.seg_v[s];
+      beg = A.pout.seg_beg[s];
+      int64_t rend = A.out_ptr[u + 1];
+      end = beg + kSeg < rend ? beg + kSeg : rend;
+    }
     uint64_t f = fr[u];
-    int64_t beg = A.out_ptr[u], end = A.out_ptr[u + 1];
-    if (lane == 0) scanned += (unsigned long long)(end - beg);
+    if (!f) continue;
+    if (lane == 0) {
+      scanned += (unsigned long long)(end - beg);
+      if (it < nq || beg == A.out_ptr[u]) rows++;
+    }
     for (int64_t e0 = beg; e0 < end; e0 += kWave) {
       int64_t e = e0 + lane;
       bool first = false;
       int32_t w = 0;
       if (e < end) {
         w = A.out_idx[e];
-        uint64_t nv = f & ~A.vis[w];
-        if (nv) {
-          uint64_t old = atomicOr((unsigned long long *)&nx[w], (unsigned long long)nv);
-          first = (old == 0);
+        if (phase == 0) {
+          uint64_t nv = f & ~A.vis[w];
+          if (nv) {
+            uint64_t old = atomicOr((unsigned long long *)&nx[w], (unsigned long long)nv);
+            first = (old == 0);
+          }
+        } else {
+          uint64_t nv = f & nx[w];
+          while (nv) {
+            int l = __ffsll((unsigned long long)nv) - 1;
+            nv &= nv - 1;
+            atomicMin((uint32_t *)&A.parents[(int64_t)l * A.n + w], (uint32_t)u);
+          }
         }
       }
-      int64_t slot = wave_append(tail, first);
-      if (first) QN[slot] = w;
+      if (phase == 0) {
+        int64_t slot = wave_append(tail, first);
+        if (first) QN[slot] = w;
+      }
     }
   }
   unsigned long long s = ms_block_sum(scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  s = ms_block_sum(rows, red);
+  if (phase == 0 && threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
-// Parent/level resolution for the vertices a push round reached.
-__global__ void __launch_bounds__(kMsBlock) ms_resolve(MsArgs A, int d) {
+// Between the push passes: mark new pairs (parent slot = +inf for the
+// atomicMin of pass 2, level = d+1), fold them into vis.
+__global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int d) {
   MsCtrl *c = A.ctrl;
-  if (c->done || c->dir != 0) return;
   __shared__ unsigned long long red[kMsWaves];
-  const uint64_t *fr = A.fw[d & 1];
   const uint64_t *nx = A.fw[(d + 1) & 1];
   const int32_t *QN = A.q[(d + 1) & 1];
   const int64_t nq = (int64_t)c->qtail[(d + 1) & 1];
-  unsigned long long scanned = 0, mf = 0;
+  unsigned long long mf = 0, pairs = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
        i += (int64_t)gridDim.x * blockDim.x) {
     int32_t v = QN[i];
     uint64_t nw = nx[v];
-    ms_pull_vertex(A, fr, v, nw, (int16_t)(d + 1), scanned);
+    uint64_t lanes = nw;
+    while (lanes) {
+      int l = __ffsll((unsigned long long)lanes) - 1;
+      lanes &= lanes - 1;
+      int64_t o = (int64_t)l * A.n + v;
+      A.parents[o] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
+      A.levels[o] = (int16_t)(d + 1);
+    }
     A.vis[v] |= nw;
     mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
+    pairs += (unsigned long long)__popcll(nw);
   }
-  unsigned long long s = ms_block_sum(scanned, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
-  s = ms_block_sum(mf, red);
+  unsigned long long s = ms_block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
-}
-
-// Pull: every vertex with an unreached lane looks for frontier neighbours.
-__global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
-  MsCtrl *c = A.ctrl;
-  if (c->done || c->dir != 1) return;
-  __shared__ unsigned long long red[kMsWaves];
-  const uint64_t *fr = A.fw[d & 1];
-  uint64_t *nx = A.fw[(d + 1) & 1];
-  int32_t *QN = A.q[(d + 1) & 1];
-  unsigned long long *tail = &c->qtail[(d + 1) & 1];
-  unsigned long long scanned = 0, mf = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // Whole wavefronts iterate together so wave_append stays convergent.
-  const int64_t nround = ((A.n + stride - 1) / stride) * stride;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nround; v += stride) {
-    uint64_t got = 0, vw = 0;
-    if (v < A.n) {
-      vw = A.vis[v];
-      uint64_t need = A.act & ~vw;
-      if (need) got = ms_pull_vertex(A, fr, v, need, (int16_t)(d + 1), scanned);
-    }
-    bool any = got != 0;
-    if (any) {
-      nx[v] = got;
-      A.vis[v] = vw | got;
-      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
-    }
-    int64_t slot = wave_append(tail, any);
-    if (any) QN[slot] = (int32_t)v;
-  }
-  unsigned long long s = ms_block_sum(scanned, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
-  s = ms_block_sum(mf, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  s = ms_block_sum(pairs, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->pairs, s);
 }
 
 // Clears the level-d frontier words (listed in Q[d]) so that buffer is zero
 // when it becomes the next-words buffer of round d+1.
 __global__ void __launch_bounds__(kMsBlock) ms_clear(MsArgs A, int d) {
   MsCtrl *c = A.ctrl;
-  if (c->done) return;
   uint64_t *fr = A.fw[d & 1];
   const int32_t *Q = A.q[d & 1];
   const int64_t nq = (int64_t)c->qtail[d & 1];
@@ -323,21 +615,18 @@ __global__ void __launch_bounds__(kMsBlock) ms_clear(MsArgs A, int d) {
 __global__ void ms_decide(MsArgs A, int d) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   MsCtrl *c = A.ctrl;
-  if (c->done) return;
   const long long nf = (long long)c->qtail[(d + 1) & 1];
-  unsigned long long now = wall_clock64();
   if (d < kMsHist) {
     A.hist->nf[d] = nf;
     A.hist->scanned[d] = (long long)c->scanned;
+    A.hist->rows[d] = (long long)c->rows;
+    A.hist->pairs[d] = (long long)c->pairs;
     A.hist->dir[d] = c->dir;
-    A.hist->t[d] = now;
   }
   c->rounds = d + 1;
-  c->t_end = now;
   c->qtail[d & 1] = 0;
   const long long mf = (long long)c->mf;
-  c->mf = 0;
-  c->scanned = 0;
+  c->mf = c->scanned = c->rows = c->pairs = 0;
   if (nf == 0) {
     c->done = 1;
     return;
@@ -424,7 +713,8 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (o.max_levels > 32767) o.max_levels = 32767;  // int16 levels
   CG_HIP(hipSetDevice(g->device));
   MsWorkspace ws;
-  int rc = ensure_wsm(g, &ws);
+  MsGraphData *md = nullptr;
+  int rc = ensure_wsm(g, &ws, &md);
   if (rc) return rc;
   hipStream_t st = g->stream;
 
@@ -433,7 +723,7 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   int16_t *dlev = levels;
   void *tmp = nullptr;
   if (!on_device) {
-    CG_HIP(hipMalloc(&tmp, plane * (sizeof(int32_t) + sizeof(int16_t)) + 256));
+    CG_HIP(hipMalloc(&tmp, al(plane * sizeof(int32_t)) + plane * sizeof(int16_t) + 256));
     dpar = (int32_t *)tmp;
     dlev = (int16_t *)((char *)tmp + al(plane * sizeof(int32_t)));
   }
@@ -452,41 +742,66 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   A.q[1] = ws.q[1];
   A.ctrl = ws.ctrl;
   A.hist = ws.hist;
+  A.pin = md->pin;
+  A.pout = md->pout;
+  A.cand = ws.cand;
+  A.bgot = ws.bgot;
   A.n = g->n;
   A.nroots = nroots;
   A.act = nroots == 64 ? ~0ull : ((1ull << nroots) - 1ull);
 
   const int blocks = grid_blocks(g->device, o.grid_mult);
+  std::vector<hipEvent_t> evs;
+  auto new_event = [&](hipEvent_t *e) -> hipError_t {
+    hipError_t r = hipEventCreate(e);
+    if (r == hipSuccess) evs.push_back(*e);
+    return r;
+  };
+  auto cleanup = [&]() {
+    for (hipEvent_t e : evs) hipEventDestroy(e);
+    if (tmp) hipFree(tmp);
+  };
+  hipEvent_t e0, e1;
   hipError_t e = hipMemcpyAsync(ws.roots, roots, sizeof(int64_t) * nroots, hipMemcpyHostToDevice, st);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (e == hipSuccess) e = hipEventCreate(&e0);
-  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = new_event(&e0);
+  if (e == hipSuccess) e = new_event(&e1);
   if (e == hipSuccess) e = hipEventRecord(e0, st);
   if (e != hipSuccess) {
-    if (tmp) hipFree(tmp);
+    cleanup();
     return hip_fail(e, "msbfs setup", __FILE__, __LINE__);
   }
   ms_init_fill<<<blocks, kMsBlock, 0, st>>>(A);
   ms_init_roots<<<1, 64, 0, st>>>(A, ws.roots, o.direction, o.alpha, o.beta, (long long)o.tau,
                                   (long long)o.max_levels, (long long)g->nnz);
-  int64_t d = 0;
-  int chunk = o.rounds_hint > 0 ? o.rounds_hint : 12;
   MsCtrl hc;
-  for (;;) {
-    for (int k = 0; k < chunk; k++, d++) {
-      ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
-      ms_resolve<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
+  e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  // One host poll per round: only the chosen direction's kernels launch,
+  // and each round's traversal kernels are bracketed by a pair of events.
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> rev;
+  std::vector<int> rdir;
+  int64_t d = 0;
+  while (e == hipSuccess && !hc.done) {
+    hipEvent_t ra, rb;
+    if ((e = new_event(&ra)) != hipSuccess || (e = new_event(&rb)) != hipSuccess) break;
+    hipEventRecord(ra, st);
+    if (hc.dir == 1) {
       ms_pull<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
-      ms_clear<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
-      ms_decide<<<1, 64, 0, st>>>(A, (int)d);
+      if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, (int)d);
+    } else {
+      ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 0);
+      ms_prep<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
+      ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 1);
     }
+    hipEventRecord(rb, st);
+    rev.push_back({ra, rb});
+    rdir.push_back(hc.dir);
+    ms_clear<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
+    ms_decide<<<1, 64, 0, st>>>(A, (int)d);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) break;
-    if (hc.done) break;
-    if (d > o.max_levels + chunk) break;
-    chunk = 4;
+    d++;
   }
   // The final pass belongs to producing the outputs, so e1 follows it.
   if (e == hipSuccess) {
@@ -497,10 +812,8 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ms = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   if (e != hipSuccess || hc.status) {
-    if (tmp) hipFree(tmp);
+    cleanup();
     if (e != hipSuccess) return hip_fail(e, "msbfs rounds", __FILE__, __LINE__);
     return set_error(hc.status, "MS-BFS exceeded max_levels (int16 levels cap at 32767)");
   }
@@ -508,24 +821,23 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   double out_ms = 0;
   if (!on_device) {
     hipEvent_t o0, o1;
-    hipEventCreate(&o0);
-    hipEventCreate(&o1);
-    hipEventRecord(o0, st);
-    e = hipMemcpyAsync(parents, dpar, plane * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    e = new_event(&o0);
+    if (e == hipSuccess) e = new_event(&o1);
+    if (e == hipSuccess) e = hipEventRecord(o0, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(parents, dpar, plane * sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(levels, dlev, plane * sizeof(int16_t), hipMemcpyDeviceToHost, st);
-    hipEventRecord(o1, st);
+    if (e == hipSuccess) e = hipEventRecord(o1, st);
     if (e == hipSuccess) e = hipEventSynchronize(o1);
     float oms = 0;
-    hipEventElapsedTime(&oms, o0, o1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&oms, o0, o1);
     out_ms = oms;
-    hipEventDestroy(o0);
-    hipEventDestroy(o1);
-    hipFree(tmp);
-    if (e != hipSuccess) return hip_fail(e, "msbfs output copy", __FILE__, __LINE__);
+    if (e != hipSuccess) {
+      cleanup();
+      return hip_fail(e, "msbfs output copy", __FILE__, __LINE__);
+    }
   }
 
   if (stats) {
-    const double hz = wall_clock_hz(g->device);
     stats->rounds_completed = hc.rounds;
     stats->total_ms = ms;
     stats->output_ms = out_ms;
@@ -535,15 +847,22 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
     if (stats->max_hist < nh) nh = stats->max_hist;
     if (nh > 0) {
       std::vector<char> hb(sizeof(MsHist));
-      CG_HIP(hipMemcpy(hb.data(), A.hist, sizeof(MsHist), hipMemcpyDeviceToHost));
+      e = hipMemcpy(hb.data(), A.hist, sizeof(MsHist), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "msbfs stats", __FILE__, __LINE__);
+      }
       const MsHist *H = (const MsHist *)hb.data();
       for (int64_t r = 0; r < nh; r++) {
         if (stats->nf_history) stats->nf_history[r] = H->nf[r];
         if (stats->direction_history) stats->direction_history[r] = H->dir[r];
         if (stats->edges_scanned) stats->edges_scanned[r] = H->scanned[r];
-        if (stats->level_ms) {
-          unsigned long long prev = r == 0 ? hc.t0 : H->t[r - 1];
-          stats->level_ms[r] = (double)(H->t[r] - prev) * 1e3 / hz;
+        if (stats->rows_scanned) stats->rows_scanned[r] = H->rows[r];
+        if (stats->pairs_found) stats->pairs_found[r] = H->pairs[r];
+        if (stats->level_ms && r < (int64_t)rev.size()) {
+          float rm = 0;
+          hipEventElapsedTime(&rm, rev[r].first, rev[r].second);
+          stats->level_ms[r] = rm;
         }
       }
     }
@@ -552,10 +871,15 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
     unsigned long long m[64];
     ms_zero<<<1, 64, 0, st>>>(ws.mlane, 64);
     ms_count<<<grid_blocks(g->device, 2), kMsBlock, 0, st>>>(ws.vis, g->m_weight, g->n, nroots, ws.mlane);
-    CG_HIP(hipGetLastError());
-    CG_HIP(hipMemcpyAsync(m, ws.mlane, sizeof(m), hipMemcpyDeviceToHost, st));
-    CG_HIP(hipStreamSynchronize(st));
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(m, ws.mlane, sizeof(m), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      cleanup();
+      return hip_fail(e, "msbfs m count", __FILE__, __LINE__);
+    }
     for (int l = 0; l < nroots; l++) m_lanes[l] = (int64_t)m[l];
   }
+  cleanup();
   return CGBFS_OK;
 }

@@ -33,6 +33,8 @@ class SearchStats:
   nf_history: np.ndarray
   direction_history: np.ndarray
   edges_scanned: np.ndarray
+  rows_scanned: np.ndarray
+  pairs_found: np.ndarray
   level_ms: np.ndarray
   visited_count: int = -1
   m_edges: int = -1
@@ -75,12 +77,16 @@ class _StatsBuf:
     self.nf = np.zeros(cap, np.int64)
     self.dir = np.zeros(cap, np.int32)
     self.scanned = np.zeros(cap, np.int64)
+    self.rows = np.zeros(cap, np.int64)
+    self.pairs = np.zeros(cap, np.int64)
     self.level_ms = np.zeros(cap, np.float64)
     s = _lib.Stats()
     s.max_hist = cap
     s.nf_history = self.nf.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     s.direction_history = self.dir.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     s.edges_scanned = self.scanned.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    s.rows_scanned = self.rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    s.pairs_found = self.pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     s.level_ms = self.level_ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     self.s = s
 
@@ -88,7 +94,8 @@ class _StatsBuf:
     r = min(int(self.s.rounds_completed), len(self.nf))
     return SearchStats(int(self.s.rounds_completed), float(self.s.total_ms),
                        float(self.s.output_ms), self.nf[:r].copy(), self.dir[:r].copy(),
-                       self.scanned[:r].copy(), self.level_ms[:r].copy(),
+                       self.scanned[:r].copy(), self.rows[:r].copy(), self.pairs[:r].copy(),
+                       self.level_ms[:r].copy(),
                        int(self.s.visited_count), int(self.s.m_edges))
 
 

@@ -72,7 +72,9 @@ typedef struct {
   int64_t *nf_history;
   int32_t *direction_history;
   int64_t *edges_scanned;   /* adjacency entries read in round r          */
-  double *level_ms;         /* device time of round r (device wall clock) */
+  int64_t *rows_scanned;    /* vertices whose adjacency round r opened     */
+  int64_t *pairs_found;     /* (lane, vertex) pairs reached in round r     */
+  double *level_ms;         /* round r's traversal kernels (HIP events)   */
   double total_ms;          /* search time on the graph's stream (HIP events):
                                init -> parents/levels resident in HBM      */
   double output_ms;         /* device->host output copy (host outputs)     */

@@ -199,14 +199,15 @@ def scipy_levels(g, source):
   return depth
 
 
-def spmv_bfs(g, source, tau=None):
+def spmv_bfs(g, source, tau=None, matrix=None):
   """The reference round loop as boolean SpMV (bool_pe.csl:865-1156): each
   round y = M x over the boolean semiring, newly = y & ~visited, stop when
   newly is empty.  Returns (levels, nf_history, direction_history,
   rounds_completed).  tau: optional one-way switch threshold, recorded in
   direction_history exactly as term_col_bcast_done does (the product is the
-  same either way)."""
-  M = g.matrix().astype(np.int32)
+  same either way).  matrix: g.matrix().astype(np.int32), prebuilt so a
+  timed caller can leave construction out, as the device search does."""
+  M = matrix if matrix is not None else g.matrix().astype(np.int32)
   x = np.zeros(g.n, dtype=bool)
   x[source] = True
   visited = x.copy()
