@@ -9,8 +9,9 @@
 // in-neighbour one level closer.
 //
 // Work tiers (RMAT degrees span 1 .. ~10^6):
-//   * degree <  64      one thread per vertex;
-//   * degree <  kSeg    one wavefront per vertex, 64 neighbours per step;
+//   * degree <  kSeg    a wavefront takes 64 consecutive vertices and walks
+//                       the concatenation of their rows, 64 entries per step
+//                       (pull; push takes one vertex per wavefront);
 //   * degree >= kSeg    the adjacency is cut into kSeg-entry segments listed
 //                       once per graph; each segment is one wavefront's work
 //                       item, and segments of one vertex combine per lane
@@ -41,7 +42,6 @@ constexpr int kMsBlock = 256;
 constexpr int kMsWaves = kMsBlock / kWave;
 constexpr int kMsHist = 4096;
 constexpr int64_t kSeg = 1024;      // segment length and wavefront-tier cap
-constexpr int64_t kThreadDeg = 64;  // thread tier below this degree
 
 int grid_blocks(int device, int mult);
 void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o);
@@ -387,47 +387,78 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t wave = gtid / kWave, nwaves = stride / kWave;
-  // Chunks of 64 consecutive vertices per wavefront.
+  // Chunks of 64 consecutive vertices per wavefront.  The rows of the chunk
+  // that still need lanes (big rows excluded) are laid end to end in a
+  // virtual edge range [0, total); each step every lane takes one virtual
+  // entry, finds its row by binary search over the rows' virtual starts,
+  // gathers the frontier word, and a segmented prefix-OR (segments = rows)
+  // gives each (row, lane) its first -- lowest-id -- hit.  Row state (got)
+  // lives in the lane that owns the row.
   const int64_t nch = (A.n + kWave - 1) / kWave;
   for (int64_t ch = wave; ch < nch; ch += nwaves) {
     const int64_t v = ch * kWave + lane;
     uint64_t vw = 0, need = 0;
-    int64_t beg = 0, end = 0;
+    int64_t rb = 0;
+    int32_t len = 0;
     if (v < A.n && !A.pin.is_big[v]) {
       vw = A.vis[v];
       need = A.act & ~vw;
       if (need) {
-        beg = A.in_ptr[v];
-        end = A.in_ptr[v + 1];
+        rb = A.in_ptr[v];
+        len = (int32_t)(A.in_ptr[v + 1] - rb);  // < kSeg for non-big rows
       }
     }
-    const int64_t deg = end - beg;
+    if (len) rows++;
+    // exclusive prefix sum of row lengths -> virtual starts
+    int32_t incl = len;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      int32_t y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    const int32_t vstart = incl - len;
+    const int32_t total = __shfl(incl, kWave - 1);
     uint64_t got = 0;
-    if (need && deg < kThreadDeg) {
-      rows++;
-      for (int64_t e = beg; e < end; e++) {
-        int32_t u = A.in_idx[e];
-        uint64_t h = fr[u] & need & ~got;
+    for (int32_t base = 0; base < total; base += kWave) {
+      const int32_t ve = base + lane;
+      // row r: the largest lane with vstart <= ve (rows of length 0 share
+      // the next row's start, so r always has entries)
+      int r = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        int cand = r + step;
+        int32_t sv = __shfl(vstart, cand);
+        if (sv <= ve) r = cand;
+      }
+      const int32_t rs = __shfl(vstart, r);
+      const int64_t rbase = __shfl(rb, r);
+      const uint64_t nr = shfl64(need, r);
+      const uint64_t gr = shfl64(got, r);
+      uint64_t h = 0;
+      int32_t u = 0;
+      if (ve < total && (nr & ~gr)) {
+        u = A.in_idx[rbase + (ve - rs)];
+        h = fr[u] & nr & ~gr;
         scanned++;
-        if (h) {
-          ms_emit(A, h, u, v, lev);
-          got |= h;
-          if (got == need) break;
-        }
       }
-    }
-    uint64_t wt = ballot64(need && deg >= kThreadDeg);
-    while (wt) {
-      int j = __ffsll((unsigned long long)wt) - 1;
-      wt &= wt - 1;
-      int64_t vj = ch * kWave + j;
-      uint64_t nj = shfl64(need, j);
-      int64_t bj = __shfl(beg, j), ej = __shfl(end, j);
-      uint64_t gj = ms_wave_pull(A, fr, vj, bj, ej, nj, lev, nullptr, scanned);
-      if (lane == j) {
-        got = gj;
-        rows++;
+      // segmented inclusive prefix-OR; the segment of lane starts at head
+      const int head = rs > base ? rs - base : 0;
+      uint64_t x = h;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        uint64_t y = shfl_up64(x, off);
+        if (lane - off >= head) x |= y;
       }
+      uint64_t prev = shfl_up64(x, 1);
+      uint64_t mine = h & ~(lane > head ? prev : 0ull);
+      if (mine) ms_emit(A, mine, u, ch * kWave + r, lev);
+      // each row owner collects its row's OR over this step
+      const int32_t my_end = vstart + len;
+      int last = (my_end < base + kWave ? my_end : base + kWave) - 1 - base;
+      bool hit_step = len && vstart < base + kWave && my_end > base;
+      uint64_t tot = shfl64(x, hit_step ? last : lane);
+      if (hit_step) got |= tot;
+      if (!__any(len && (need & ~got))) break;  // every row of the chunk satisfied
     }
     bool any = got != 0;
     if (any) {
