@@ -58,6 +58,9 @@ def parse_args():
   p.add_argument("--cpu-roots", type=int, default=1,
                  help="roots timed for the CPU baseline (0 disables it)")
   p.add_argument("--no-per-root", action="store_true", help="skip the one-root-at-a-time leg")
+  p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                 help="also run one device-to-device copy of MIB MiB (a known-size "
+                      "reference kernel for scaling HBM byte counters)")
   return p.parse_args()
 
 
@@ -81,6 +84,13 @@ def main():
   cands = np.nonzero(deg > 0)[0]  # RMAT output has no self-loops
   roots = np.random.default_rng(args.root_seed).choice(cands, size=args.nroots, replace=False)
 
+  if args.calibrate_copy:
+    src = torch.ones(args.calibrate_copy << 18, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+    del src, dst
   par = torch.empty(args.nroots * n, dtype=torch.int32, device="cuda")
   lev = torch.empty(args.nroots * n, dtype=torch.int16, device="cuda")
   for i in range(args.warmup):

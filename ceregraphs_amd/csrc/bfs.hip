@@ -30,7 +30,8 @@ namespace cg {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kHistCap = 4096;        // rounds recorded in the history arrays
-constexpr int64_t kBigDegree = 8192;  // push: whole-grid expansion at/above this
+constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/above this
+constexpr int kBigLds = 4096;         // push: big-list prefix entries staged per pass
 constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this
 constexpr int64_t kPullWaveDegree = 128;  // pull: wavefront scan at/above this
 constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
@@ -38,6 +39,7 @@ constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
 struct Ctrl {
   unsigned long long qtail[2];  // queue lengths, Q[0] and Q[1]
   unsigned long long big_tail;  // push: count of whole-grid vertices
+  unsigned long long big_edges; // push: their total degree
   unsigned long long nf;        // vertices discovered this round
   unsigned long long mf;        // out-degree sum of the discovered vertices
   unsigned long long in_sum;    // in-degree sum of the discovered vertices
@@ -45,16 +47,19 @@ struct Ctrl {
   unsigned long long rows;      // adjacency lists opened this round
   long long mu;                 // in-edges of still-unvisited vertices
   long long prev_nf;
-  long long tau;
-  long long max_levels;
   long long rounds;
-  double alpha, beta;
   int dir;  // direction of the current round: 0 push, 1 pull
-  int mode;
   int done;
   int status;
   unsigned long long t0;
   unsigned long long t_end;
+};
+
+// Per-search constants, passed by value to the kernels that need them.
+struct BfsParams {
+  int mode;
+  double alpha, beta;
+  long long tau, max_levels;
 };
 
 struct Hist {
@@ -72,6 +77,7 @@ struct Workspace {
   uint32_t *fb[3] = {nullptr, nullptr, nullptr};
   int32_t *q[2] = {nullptr, nullptr};
   int32_t *bigq = nullptr;
+  int64_t *bigpre = nullptr;  // exclusive degree prefix of bigq
   int32_t *levels = nullptr;   // internal outputs (host-output calls)
   uint32_t *parents = nullptr;
   Ctrl *ctrl = nullptr;
@@ -86,7 +92,7 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
   const int64_t nwords = ((n + 63) / 64) * 2;
   size_t bm = align_up(sizeof(uint32_t) * nwords);
   size_t qb = align_up(sizeof(int32_t) * n);
-  size_t total = bm * 4 + qb * 3 + align_up(sizeof(int32_t) * n) + align_up(sizeof(uint32_t) * n) +
+  size_t total = bm * 4 + qb * 3 + align_up(sizeof(int64_t) * (n + 1)) + align_up(sizeof(int32_t) * n) + align_up(sizeof(uint32_t) * n) +
                  align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64);
   if (g->ws1_bytes < total) {
     if (g->ws1) hipFree(g->ws1);
@@ -102,6 +108,7 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
   for (int i = 0; i < 3; i++) { w->fb[i] = (uint32_t *)p; p += bm; }
   for (int i = 0; i < 2; i++) { w->q[i] = (int32_t *)p; p += qb; }
   w->bigq = (int32_t *)p; p += qb;
+  w->bigpre = (int64_t *)p; p += align_up(sizeof(int64_t) * (n + 1));
   w->levels = (int32_t *)p; p += align_up(sizeof(int32_t) * n);
   w->parents = (uint32_t *)p; p += align_up(sizeof(uint32_t) * n);
   w->ctrl = (Ctrl *)p; p += align_up(sizeof(Ctrl));
@@ -137,6 +144,7 @@ struct LevelArgs {
   uint32_t *fb[3];
   int32_t *q[2];
   int32_t *bigq;
+  int64_t *bigpre;
   Ctrl *ctrl;
   Hist *hist;
   int64_t n;
@@ -161,8 +169,7 @@ __global__ void __launch_bounds__(kBlock) bfs_init_fill(LevelArgs A) {
   }
 }
 
-__global__ void bfs_init_source(LevelArgs A, int64_t s, int mode, double alpha, double beta,
-                                long long tau, long long max_levels, long long total_in) {
+__global__ void bfs_init_source(LevelArgs A, int64_t s, BfsParams P, long long total_in) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   A.levels[s] = 0;
   A.parents[s] = (uint32_t)s;
@@ -170,23 +177,24 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, int mode, double alpha, 
   A.fb[0][s >> 5] |= 1u << (s & 31);
   A.q[0][0] = (int32_t)s;
   Ctrl *c = A.ctrl;
-  c->qtail[0] = 1;
-  c->qtail[1] = 0;
-  c->big_tail = 0;
-  c->nf = c->mf = c->in_sum = c->scanned = c->rows = 0;
-  c->mu = total_in - (A.in_ptr[s + 1] - A.in_ptr[s]);
-  c->prev_nf = 1;
-  c->tau = tau;
-  c->max_levels = max_levels;
-  c->rounds = 0;
-  c->alpha = alpha;
-  c->beta = beta;
-  c->mode = mode;
-  c->dir = (mode == CGBFS_DIR_BOTTOMUP) ? 1 : 0;  // round 0 is push (bool_pe.csl:714-719)
-  c->done = 0;
-  c->status = 0;
-  c->t0 = wall_clock64();
-  c->t_end = c->t0;
+  st_ctrl(&c->qtail[0], 1ull);
+  st_ctrl(&c->qtail[1], 0ull);
+  st_ctrl(&c->big_tail, 0ull);
+  st_ctrl(&c->nf, 0ull);
+  st_ctrl(&c->mf, 0ull);
+  st_ctrl(&c->in_sum, 0ull);
+  st_ctrl(&c->scanned, 0ull);
+  st_ctrl(&c->rows, 0ull);
+  st_ctrl(&c->mu, (long long)(total_in - (A.in_ptr[s + 1] - A.in_ptr[s])));
+  st_ctrl(&c->prev_nf, 1ll);
+  st_ctrl(&c->rounds, 0ll);
+  // round 0 is push (bool_pe.csl:714-719) unless every round pulls
+  st_ctrl(&c->dir, P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0);
+  st_ctrl(&c->done, 0);
+  st_ctrl(&c->status, 0);
+  const unsigned long long t0 = wall_clock64();
+  st_ctrl(&c->t0, t0);
+  st_ctrl(&c->t_end, t0);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,7 +244,7 @@ __device__ __forceinline__ void push_flush(PushLocal &L, unsigned long long *gta
 
 __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
   Ctrl *c = A.ctrl;
-  if (c->done || c->dir != 0) return;
+  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
   __shared__ int32_t lq[kLdsQueue];
   __shared__ unsigned lq_n;
   __shared__ unsigned lds_base;
@@ -250,7 +258,7 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
   unsigned long long *gtail = &c->qtail[nxt];
   uint32_t *fnext = A.fb[(d + 1) % 3];
   uint32_t *fclear = A.fb[(d + 2) % 3];
-  const int64_t nq = (int64_t)c->qtail[cur];
+  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[cur]);
 
   int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t gstride = (int64_t)gridDim.x * blockDim.x;
@@ -276,7 +284,7 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
       L.scanned += (unsigned long long)deg;
     }
     if (mid) L.scanned += (unsigned long long)deg;
-    if (valid) L.rows++;
+    if (valid && !big) L.rows++;
     uint64_t mm = ballot64(mid);
     while (mm) {
       int j = __ffsll((unsigned long long)mm) - 1;
@@ -298,17 +306,64 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
   if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
-// Whole-grid expansion of the push round's highest-degree vertices: every
-// thread of the grid takes a strided slice of each listed vertex's edges.
+// Degree prefix of the push round's big list (one block): bigpre[j] = sum of
+// the degrees of bigq[0..j), bigpre[nb] = big_edges.
+__global__ void __launch_bounds__(1024) bfs_push_scan(LevelArgs A) {
+  Ctrl *c = A.ctrl;
+  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
+  const int64_t nb = (int64_t)ld_ctrl(&c->big_tail);
+  if (nb == 0) return;
+  __shared__ long long wsum[1024 / kWave];
+  __shared__ long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  for (int64_t b0 = 0; b0 < nb; b0 += blockDim.x) {
+    int64_t i = b0 + threadIdx.x;
+    long long deg = 0;
+    if (i < nb) {
+      int32_t u = A.bigq[i];
+      deg = (long long)(A.out_ptr[u + 1] - A.out_ptr[u]);
+    }
+    long long x = deg;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      long long y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    long long before = carry;
+    for (int k = 0; k < w; k++) before += wsum[k];
+    if (i < nb) A.bigpre[i] = before + x - deg;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) {
+      long long t = 0;
+      for (int k = 0; k < (int)(blockDim.x / kWave); k++) t += wsum[k];
+      carry += t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    A.bigpre[nb] = carry;
+    st_ctrl(&c->big_edges, (unsigned long long)carry);
+  }
+}
+
+// Whole-grid flat expansion of the big list: the big vertices' edges are
+// numbered 0..big_edges-1 in list order and every thread takes flat indices
+// gtid, gtid + stride, ...; the owning vertex comes from a binary search
+// over the degree prefix staged in LDS (kBigLds entries per pass).
 __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
   Ctrl *c = A.ctrl;
-  if (c->done || c->dir != 0) return;
-  const int64_t nb = (int64_t)c->big_tail;
+  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
+  const int64_t nb = (int64_t)ld_ctrl(&c->big_tail);
   if (nb == 0) return;
   __shared__ int32_t lq[kLdsQueue];
   __shared__ unsigned lq_n;
   __shared__ unsigned lds_base;
   __shared__ unsigned long long red[kWavesPerBlock];
+  __shared__ long long pre[kBigLds + 1];
   if (threadIdx.x == 0) lq_n = 0;
   __syncthreads();
   PushLocal L{lq, &lq_n, 0, 0, 0, 0};
@@ -316,13 +371,29 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
   int32_t *QN = A.q[nxt];
   unsigned long long *gtail = &c->qtail[nxt];
   uint32_t *fnext = A.fb[(d + 1) % 3];
-  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t j = 0; j < nb; j++) {
-    int32_t u = A.bigq[j];
-    int64_t beg = A.out_ptr[u], end = A.out_ptr[u + 1];
-    for (int64_t e = beg + gtid; e < end; e += gstride) push_visit(A, d, (uint32_t)u, A.out_idx[e], fnext, L, gtail, QN);
-    if (gtid == 0) L.scanned += (unsigned long long)(end - beg);
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j0 = 0; j0 < nb; j0 += kBigLds) {
+    const int cnt = (int)((nb - j0) < kBigLds ? (nb - j0) : kBigLds);
+    for (int k = threadIdx.x; k <= cnt; k += blockDim.x) pre[k] = A.bigpre[j0 + k];
+    __syncthreads();
+    const long long lo = pre[0], hi = pre[cnt];
+    for (long long f = lo + gtid; f < hi; f += gstride) {
+      int a = 0, b = cnt - 1;  // largest j with pre[j] <= f
+      while (a < b) {
+        int mid = (a + b + 1) >> 1;
+        if (pre[mid] <= f) a = mid;
+        else b = mid - 1;
+      }
+      int32_t u = A.bigq[j0 + a];
+      int64_t e = A.out_ptr[u] + (f - pre[a]);
+      push_visit(A, d, (uint32_t)u, A.out_idx[e], fnext, L, gtail, QN);
+    }
+    if (gtid == 0) {
+      L.scanned += (unsigned long long)(hi - lo);
+      L.rows += (unsigned long long)cnt;
+    }
+    __syncthreads();
   }
   push_flush(L, gtail, QN, &lds_base);
   unsigned long long s;
@@ -341,13 +412,18 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
   Ctrl *c = A.ctrl;
-  if (c->done || c->dir != 1) return;
+  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 1) return;
   __shared__ unsigned long long red[kWavesPerBlock];
   const uint32_t *fcur = A.fb[d % 3];
   uint32_t *fnext = A.fb[(d + 1) % 3];
   uint32_t *fclear = A.fb[(d + 2) % 3];
   int32_t *QN = A.q[(d + 1) & 1];
   unsigned long long *gtail = &c->qtail[(d + 1) & 1];
+  __shared__ int32_t lq[kLdsQueue];
+  __shared__ unsigned lq_n;
+  __shared__ unsigned long long lq_base;
+  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kLdsQueue, gtail, QN};
+  sq.init();
   int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t w = gtid; w < A.nwords; w += gstride) fclear[w] = 0;
@@ -423,10 +499,10 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
         if (lo) { fnext[w0] = lo; A.visited[w0] = vlo | lo; }
         if (hi) { fnext[w0 + 1] = hi; A.visited[w0 + 1] = vhi | hi; }
       }
-      int64_t slot = wave_append(gtail, hit);
-      if (hit) QN[slot] = (int32_t)v;
+      sq.push(hit, (int32_t)v);
     }
   }
+  sq.flush();
   unsigned long long s;
   s = block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
@@ -441,50 +517,56 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
 // ---------------------------------------------------------------------------
 // Decide: termination, direction, history (one thread)
 // ---------------------------------------------------------------------------
-__global__ void bfs_decide(LevelArgs A, int d, int64_t n) {
+__global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Ctrl *c = A.ctrl;
-  if (c->done) return;
+  if (ld_ctrl(&c->done)) return;
   const int nxt = (d + 1) & 1;
-  const long long nf = (long long)c->qtail[nxt];
+  const int cur = ld_ctrl(&c->dir);
+  const long long nf = (long long)ld_ctrl(&c->qtail[nxt]);
+  const long long mf = (long long)ld_ctrl(&c->mf);
+  const long long mu = ld_ctrl(&c->mu) - (long long)ld_ctrl(&c->in_sum);
   unsigned long long now = wall_clock64();
   if (d < kHistCap) {
     A.hist->nf[d] = nf;
-    A.hist->scanned[d] = (long long)c->scanned;
-    A.hist->rows[d] = (long long)c->rows;
-    A.hist->dir[d] = c->dir;
+    A.hist->scanned[d] = (long long)ld_ctrl(&c->scanned);
+    A.hist->rows[d] = (long long)ld_ctrl(&c->rows);
+    A.hist->dir[d] = cur;
     A.hist->t[d] = now;
   }
-  c->rounds = d + 1;
-  c->t_end = now;
-  c->qtail[d & 1] = 0;
-  c->big_tail = 0;
-  c->mu -= (long long)c->in_sum;
-  const long long mf = (long long)c->mf;
-  c->nf = c->mf = c->in_sum = c->scanned = c->rows = 0;
+  st_ctrl(&c->rounds, (long long)(d + 1));
+  st_ctrl(&c->t_end, now);
+  st_ctrl(&c->qtail[d & 1], 0ull);
+  st_ctrl(&c->big_tail, 0ull);
+  st_ctrl(&c->mu, mu);
+  st_ctrl(&c->nf, 0ull);
+  st_ctrl(&c->mf, 0ull);
+  st_ctrl(&c->in_sum, 0ull);
+  st_ctrl(&c->scanned, 0ull);
+  st_ctrl(&c->rows, 0ull);
   if (nf == 0) {
-    c->done = 1;
+    st_ctrl(&c->done, 1);
     return;
   }
-  int dir = c->dir;
-  switch (c->mode) {
+  int dir = cur;
+  switch (P.mode) {
     case CGBFS_DIR_TOPDOWN: dir = 0; break;
     case CGBFS_DIR_BOTTOMUP: dir = 1; break;
     case CGBFS_DIR_TAU:
-      if (dir == 0 && nf > c->tau) dir = 1;  // one-way, bool_pe.csl:1078-1088
+      if (dir == 0 && nf > P.tau) dir = 1;  // one-way, bool_pe.csl:1078-1088
       break;
     default:  // Beamer et al. alpha/beta
       if (dir == 0) {
-        if ((double)mf > (double)c->mu / c->alpha) dir = 1;
+        if ((double)mf > (double)mu / P.alpha) dir = 1;
       } else {
-        if ((double)nf < (double)n / c->beta && nf < c->prev_nf) dir = 0;
+        if ((double)nf < (double)n / P.beta && nf < ld_ctrl(&c->prev_nf)) dir = 0;
       }
   }
-  c->prev_nf = nf;
-  c->dir = dir;
-  if (d + 1 >= c->max_levels) {
-    c->status = CGBFS_ETIMEOUT;
-    c->done = 1;
+  st_ctrl(&c->prev_nf, nf);
+  st_ctrl(&c->dir, dir);
+  if (d + 1 >= P.max_levels) {
+    st_ctrl(&c->status, (int)CGBFS_ETIMEOUT);
+    st_ctrl(&c->done, 1);
   }
 }
 
@@ -564,6 +646,7 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   A.q[0] = ws.q[0];
   A.q[1] = ws.q[1];
   A.bigq = ws.bigq;
+  A.bigpre = ws.bigpre;
   A.ctrl = ws.ctrl;
   A.hist = ws.hist;
   A.n = g->n;
@@ -576,8 +659,13 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   if (!e0 || !e1) return set_error(CGBFS_EHIP, "hipEventCreate failed");
   CG_HIP(hipEventRecord(e0, st));
   bfs_init_fill<<<blocks, kBlock, 0, st>>>(A);
-  bfs_init_source<<<1, 64, 0, st>>>(A, source, o.direction, o.alpha, o.beta, (long long)o.tau,
-                                    (long long)o.max_levels, (long long)g->total_out_edges);
+  BfsParams P;
+  P.mode = o.direction;
+  P.alpha = o.alpha;
+  P.beta = o.beta;
+  P.tau = (long long)o.tau;
+  P.max_levels = (long long)o.max_levels;
+  bfs_init_source<<<1, 64, 0, st>>>(A, source, P, (long long)g->total_out_edges);
   CG_HIP(hipGetLastError());
 
   // Rounds are enqueued in chunks without host synchronisation; kernels of
@@ -597,10 +685,11 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
         hipEventRecord(ra, st);
       }
       bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
+      bfs_push_scan<<<1, 1024, 0, st>>>(A);
       bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);
       bfs_pull_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
       if (rb) hipEventRecord(rb, st);
-      bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n);
+      bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n, P);
     }
     CG_HIP(hipGetLastError());
     CG_HIP(hipEventRecord(e1, st));

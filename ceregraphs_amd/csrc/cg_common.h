@@ -87,6 +87,19 @@ struct EventPool {
 // Device helpers
 // ---------------------------------------------------------------------------
 
+// Control words (queue tails, counters, flags) are updated by atomics from
+// every XCD and read by later kernels, often single-block ones.  A plain load
+// may be served from a line the reading XCD's L2 still holds, so they are
+// read and written with agent-scope atomic operations only.
+template <class T>
+__device__ __forceinline__ T ld_ctrl(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_ctrl(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -103,6 +116,56 @@ __device__ __forceinline__ int64_t wave_append(unsigned long long *tail, bool wa
   uint64_t below = mask & ((1ull << lane_id()) - 1ull);
   return (int64_t)(base + __popcll(below));
 }
+
+// Append queue staged in LDS: wavefronts reserve slots with an LDS atomic,
+// and the block publishes its entries with one global atomicAdd in flush().
+// Entries past the LDS capacity go straight to global memory (still one
+// global atomic per wavefront call).  push() is wavefront-uniform; every
+// thread of the block must reach init() and flush().
+struct StagedQueue {
+  int32_t *buf;
+  unsigned *cnt;
+  unsigned long long *base;
+  unsigned cap;
+  unsigned long long *gtail;
+  int32_t *gq;
+
+  __device__ void init() {
+    if (threadIdx.x == 0) *cnt = 0;
+    __syncthreads();
+  }
+
+  __device__ void push(bool want, int32_t val) {
+    uint64_t m = __ballot(want);
+    if (m == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned b = 0;
+    if (lane == leader) b = atomicAdd(cnt, (unsigned)__popcll(m));
+    b = __shfl(b, leader);
+    unsigned my = b + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    bool spill = want && my >= cap;
+    if (want && !spill) buf[my] = val;
+    uint64_t sm = __ballot(spill);
+    if (sm) {
+      const int sl = __ffsll((unsigned long long)sm) - 1;
+      unsigned long long gb = 0;
+      if (lane == sl) gb = atomicAdd(gtail, (unsigned long long)__popcll(sm));
+      gb = __shfl(gb, sl);
+      if (spill) gq[gb + __popcll(sm & ((1ull << lane) - 1ull))] = val;
+    }
+  }
+
+  __device__ void flush() {
+    __syncthreads();
+    unsigned n = *cnt < cap ? *cnt : cap;
+    if (threadIdx.x == 0) *base = n ? atomicAdd(gtail, (unsigned long long)n) : 0ull;
+    __syncthreads();
+    const unsigned long long b = *base;
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) gq[b + i] = buf[i];
+    __syncthreads();
+  }
+};
 
 // Sums a 64-bit value across the wavefront.
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {

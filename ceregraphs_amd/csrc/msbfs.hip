@@ -30,6 +30,8 @@
 // Outputs parents[l][v] (int32) and levels[l][v] (int16) are written once
 // per (lane, vertex): at discovery, or by the final pass for unreached
 // pairs.  No initial fill of the 6 B x 64 x V output arrays is needed.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -42,26 +44,31 @@ constexpr int kMsBlock = 256;
 constexpr int kMsWaves = kMsBlock / kWave;
 constexpr int kMsHist = 4096;
 constexpr int64_t kSeg = 1024;      // segment length and wavefront-tier cap
+constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
+constexpr int kUnroll = 4;          // pull steps whose loads are issued together
 
 int grid_blocks(int device, int mult);
 void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o);
 
 struct MsCtrl {
   unsigned long long qtail[2];
+  unsigned long long nf;       // vertices newly active after a pull round
   unsigned long long mf;       // out-degree sum of the next active set
   unsigned long long scanned;  // adjacency entries read this round
   unsigned long long rows;     // adjacency lists opened this round
   unsigned long long pairs;    // (lane, vertex) pairs reached this round
   long long prev_nf;
-  long long tau;
-  long long max_levels;
   long long rounds;
-  double alpha, beta;
-  long long nnz;
   int dir;
-  int mode;
   int done;
   int status;
+};
+
+// Per-search constants, passed by value to the kernels that need them.
+struct MsParams {
+  int mode;
+  double alpha, beta;
+  long long tau, max_levels, nnz;
 };
 
 struct MsHist {
@@ -97,6 +104,12 @@ struct MsArgs {
   MsHist *hist;
   // big-vertex segments, pull (in view) and push (out view)
   SegList pin, pout;
+  // Vertex-state slot of each vertex (hub-first order) and the adjacency
+  // arrays re-expressed in slots, entry for entry; the lane words vis/fw
+  // are indexed by slot, everything else by vertex id.
+  const int32_t *slot_of;
+  const int32_t *in_slot;
+  const int32_t *out_slot;
   uint32_t *cand;    // [pin.nbig][64] per-lane first-hit candidates
   uint64_t *bgot;    // [pin.nbig] lanes found this round
   int64_t n;
@@ -109,6 +122,10 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 struct MsGraphData {
   SegList pin, pout;
   void *mem = nullptr;
+  int32_t *slot_of = nullptr;   // [n]
+  int32_t *vertex_of = nullptr; // [n], inverse of slot_of
+  int32_t *in_slot = nullptr;   // [nnz], in view
+  int32_t *out_slot = nullptr;  // [nnz], out view (== in_slot when shared)
 };
 
 struct MsWorkspace {
@@ -185,22 +202,96 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
   return CGBFS_OK;
 }
 
+// Hub-first slots: vertices ordered by out-degree (descending, ties by id).
+// An in-neighbour u is gathered once per out-edge of u, so this packs the
+// most-gathered lane words into the first few MB, where they stay cached.
+__global__ void slot_keys_kernel(const int64_t *__restrict__ out_ptr, int64_t n,
+                                 uint64_t *__restrict__ keys) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int64_t deg = out_ptr[v + 1] - out_ptr[v];
+  uint64_t rank = (uint64_t)(0x7FFFFFFFll - (deg < 0x7FFFFFFFll ? deg : 0x7FFFFFFFll));
+  keys[v] = (rank << 32) | (uint64_t)v;
+}
+
+__global__ void slot_scatter_kernel(const uint64_t *__restrict__ sorted, int64_t n,
+                                    int32_t *__restrict__ slot_of, int32_t *__restrict__ vertex_of) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t v = (int32_t)(sorted[k] & 0xFFFFFFFFull);
+  slot_of[v] = (int32_t)k;
+  vertex_of[k] = v;
+}
+
+__global__ void slot_map_kernel(const int32_t *__restrict__ idx, int64_t nnz,
+                                const int32_t *__restrict__ slot_of, int32_t *__restrict__ out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  out[e] = slot_of[idx[e]];
+}
+
+static int build_slots(DeviceGraph *g, MsGraphData *md) {
+  const int64_t n = g->n, nnz = g->nnz;
+  hipStream_t st = g->stream;
+  uint64_t *keys = nullptr, *sorted = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int rc = CGBFS_OK;
+  hipError_t e = hipMalloc(&md->slot_of, sizeof(int32_t) * n);
+  if (e == hipSuccess) e = hipMalloc(&md->vertex_of, sizeof(int32_t) * n);
+  if (e == hipSuccess) e = hipMalloc(&md->in_slot, sizeof(int32_t) * (nnz > 0 ? nnz : 1));
+  if (e == hipSuccess && g->in_idx != g->out_idx)
+    e = hipMalloc(&md->out_slot, sizeof(int32_t) * (nnz > 0 ? nnz : 1));
+  if (e == hipSuccess) e = hipMalloc(&keys, sizeof(uint64_t) * n);
+  if (e == hipSuccess) e = hipMalloc(&sorted, sizeof(uint64_t) * n);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, keys, sorted, n, 0, 64, st);
+  if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes + 16);
+  if (e == hipSuccess) {
+    slot_keys_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g->out_ptr, n, keys);
+    e = hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, keys, sorted, n, 0, 64, st);
+  }
+  if (e == hipSuccess) {
+    slot_scatter_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(sorted, n, md->slot_of,
+                                                                      md->vertex_of);
+    if (nnz > 0) {
+      slot_map_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, st>>>(g->in_idx, nnz, md->slot_of, md->in_slot);
+      if (md->out_slot)
+        slot_map_kernel<<<(unsigned)((nnz + 255) / 256), 256, 0, st>>>(g->out_idx, nnz, md->slot_of, md->out_slot);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) rc = hip_fail(e, "vertex slot build", __FILE__, __LINE__);
+  if (!md->out_slot) md->out_slot = md->in_slot;
+  hipFree(keys);
+  hipFree(sorted);
+  hipFree(tmp);
+  return rc;
+}
+
+static void free_msdata(void *q) {
+  MsGraphData *m = (MsGraphData *)q;
+  if (m->mem) hipFree(m->mem);
+  if (m->out_slot && m->out_slot != m->in_slot) hipFree(m->out_slot);
+  if (m->in_slot) hipFree(m->in_slot);
+  if (m->slot_of) hipFree(m->slot_of);
+  if (m->vertex_of) hipFree(m->vertex_of);
+  delete m;
+}
+
 static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
   const int64_t n = g->n;
   if (!g->msdata) {
     MsGraphData *md = new MsGraphData();
     int rc = build_seglists(g, md);
+    if (!rc) rc = build_slots(g, md);
     if (rc) {
-      if (md->mem) hipFree(md->mem);
-      delete md;
+      free_msdata(md);
       return rc;
     }
     g->msdata = md;
-    g->msdata_free = [](void *q) {
-      MsGraphData *m = (MsGraphData *)q;
-      if (m->mem) hipFree(m->mem);
-      delete m;
-    };
+    g->msdata_free = free_msdata;
   }
   MsGraphData *md = (MsGraphData *)g->msdata;
   *mdp = md;
@@ -268,49 +359,30 @@ __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
   return s;
 }
 
-// Writes parents/levels for lanes `lanes` of vertex v whose parent is u.
-__device__ __forceinline__ void ms_emit(const MsArgs &A, uint64_t lanes, int32_t u, int64_t v,
-                                        int16_t lev) {
-  while (lanes) {
-    int l = __ffsll((unsigned long long)lanes) - 1;
-    lanes &= lanes - 1;
-    int64_t o = (int64_t)l * A.n + v;
-    A.parents[o] = u;
-    A.levels[o] = lev;
-  }
-}
 
-// Wavefront scan of in-neighbours [beg, end) of v for lanes `need`
-// (wave-uniform arguments).  Emits first hits when `emit`, else offers them
-// to cand[] with atomicMin.  Returns the lanes found (wave-uniform).
-__device__ __forceinline__ uint64_t ms_wave_pull(const MsArgs &A, const uint64_t *fr, int64_t v,
-                                                 int64_t beg, int64_t end, uint64_t need,
-                                                 int16_t lev, uint32_t *cand,
+// Wavefront scan of in-neighbour entries [beg, end) for lanes `need`
+// (wave-uniform arguments): gathers lane words through in_slot and offers
+// each lane's first hit in the range (its original id) to cand[] with
+// atomicMin.  Returns the lanes found (wave-uniform).
+__device__ __forceinline__ uint64_t ms_wave_pull(const MsArgs &A, const uint64_t *fr, int64_t beg,
+                                                 int64_t end, uint64_t need, uint32_t *cand,
                                                  unsigned long long &scanned) {
   const int lane = lane_id();
   uint64_t got = 0;
   for (int64_t e0 = beg; e0 < end; e0 += kWave) {
     int64_t e = e0 + lane;
     uint64_t h = 0;
-    int32_t u = 0;
-    if (e < end) {
-      u = A.in_idx[e];
-      h = fr[u] & need & ~got;
-    }
+    if (e < end) h = fr[A.in_slot[e]] & need & ~got;
     uint64_t inc = wave_scan_or(h);
     uint64_t before = shfl_up64(inc, 1);
     if (lane == 0) before = 0;
     uint64_t mine = h & ~before;
     if (mine) {
-      if (cand) {
-        uint64_t b = mine;
-        while (b) {
-          int l = __ffsll((unsigned long long)b) - 1;
-          b &= b - 1;
-          atomicMin(&cand[l], (uint32_t)u);
-        }
-      } else {
-        ms_emit(A, mine, u, v, lev);
+      const uint32_t u = (uint32_t)A.in_idx[e];
+      while (mine) {
+        int l = __ffsll((unsigned long long)mine) - 1;
+        mine &= mine - 1;
+        atomicMin(&cand[l], u);
       }
     }
     got |= shfl64(inc, kWave - 1);
@@ -335,40 +407,38 @@ __global__ void __launch_bounds__(kMsBlock) ms_init_fill(MsArgs A) {
   for (int64_t i = tid; i < A.pin.nbig; i += stride) A.bgot[i] = 0;
 }
 
-__global__ void ms_init_roots(MsArgs A, const int64_t *roots, int mode, double alpha, double beta,
-                              long long tau, long long max_levels, long long nnz) {
+__global__ void ms_init_roots(MsArgs A, const int64_t *roots, MsParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long nq = 0, mf = 0;
   for (int l = 0; l < A.nroots; l++) {
     int64_t r = roots[l];
+    int32_t sr = A.slot_of[r];
     uint64_t bit = 1ull << l;
-    if (A.fw[0][r] == 0) {
+    if (A.fw[0][sr] == 0) {
       A.q[0][nq++] = (int32_t)r;
       mf += (unsigned long long)(A.out_ptr[r + 1] - A.out_ptr[r]);
     }
-    A.vis[r] |= bit;
-    A.fw[0][r] |= bit;
+    A.vis[sr] |= bit;
+    A.fw[0][sr] |= bit;
     A.parents[(int64_t)l * A.n + r] = (int32_t)r;
     A.levels[(int64_t)l * A.n + r] = 0;
   }
   MsCtrl *c = A.ctrl;
-  c->qtail[0] = nq;
-  c->qtail[1] = 0;
-  c->mf = c->scanned = c->rows = c->pairs = 0;
-  c->prev_nf = (long long)nq;
-  c->tau = tau;
-  c->max_levels = max_levels;
-  c->rounds = 0;
-  c->alpha = alpha;
-  c->beta = beta;
-  c->nnz = nnz;
-  c->mode = mode;
+  st_ctrl(&c->qtail[0], nq);
+  st_ctrl(&c->qtail[1], 0ull);
+  st_ctrl(&c->nf, 0ull);
+  st_ctrl(&c->mf, 0ull);
+  st_ctrl(&c->scanned, 0ull);
+  st_ctrl(&c->rows, 0ull);
+  st_ctrl(&c->pairs, 0ull);
+  st_ctrl(&c->prev_nf, (long long)nq);
+  st_ctrl(&c->rounds, 0ll);
   int dir = 0;
-  if (mode == CGBFS_DIR_BOTTOMUP) dir = 1;
-  else if (mode == CGBFS_DIR_BEAMER && (double)mf * alpha > (double)nnz) dir = 1;
-  c->dir = dir;
-  c->done = 0;
-  c->status = 0;
+  if (P.mode == CGBFS_DIR_BOTTOMUP) dir = 1;
+  else if (P.mode == CGBFS_DIR_BEAMER && (double)mf * P.alpha > (double)P.nnz) dir = 1;
+  st_ctrl(&c->dir, dir);
+  st_ctrl(&c->done, 0);
+  st_ctrl(&c->status, 0);
 }
 
 // Pull round over non-big vertices (thread and wavefront tiers), then the
@@ -379,14 +449,18 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
   __shared__ unsigned long long red[kMsWaves];
   const uint64_t *fr = A.fw[d & 1];
   uint64_t *nx = A.fw[(d + 1) & 1];
-  int32_t *QN = A.q[(d + 1) & 1];
-  unsigned long long *tail = &c->qtail[(d + 1) & 1];
   const int16_t lev = (int16_t)(d + 1);
   unsigned long long scanned = 0, mf = 0, rows = 0, pairs = 0;
   const int lane = lane_id();
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t wave = gtid / kWave, nwaves = stride / kWave;
+  // First hits of the current chunk, as offsets into the row (rows here are
+  // shorter than kSeg), indexed [lane bit][row] so the per-bit flush below
+  // reads one conflict-free row of LDS and stores 64 consecutive vertices.
+  __shared__ uint16_t ptile[kMsWaves][kWave][kWave];
+  uint16_t (*pt)[kWave] = ptile[threadIdx.x / kWave];
+  unsigned long long nf = 0;
   // Chunks of 64 consecutive vertices per wavefront.  The rows of the chunk
   // that still need lanes (big rows excluded) are laid end to end in a
   // virtual edge range [0, total); each step every lane takes one virtual
@@ -399,9 +473,10 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
     const int64_t v = ch * kWave + lane;
     uint64_t vw = 0, need = 0;
     int64_t rb = 0;
-    int32_t len = 0;
+    int32_t len = 0, sv_me = 0;
     if (v < A.n && !A.pin.is_big[v]) {
-      vw = A.vis[v];
+      sv_me = A.slot_of[v];
+      vw = A.vis[sv_me];
       need = A.act & ~vw;
       if (need) {
         rb = A.in_ptr[v];
@@ -419,70 +494,110 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
     const int32_t vstart = incl - len;
     const int32_t total = __shfl(incl, kWave - 1);
     uint64_t got = 0;
-    for (int32_t base = 0; base < total; base += kWave) {
-      const int32_t ve = base + lane;
-      // row r: the largest lane with vstart <= ve (rows of length 0 share
-      // the next row's start, so r always has entries)
-      int r = 0;
+    const int32_t my_end = vstart + len;
+    // kUnroll steps per pass: their index loads, then their frontier-word
+    // gathers, are all in flight together; the scans then run step by step
+    // (a row already satisfied when the pass starts issues no loads).
+    for (int32_t base0 = 0; base0 < total; base0 += kWave * kUnroll) {
+      int rk[kUnroll];
+      int32_t rsk[kUnroll], idx[kUnroll];
+      uint64_t nrk[kUnroll], raw[kUnroll];
 #pragma unroll
-      for (int step = 32; step > 0; step >>= 1) {
-        int cand = r + step;
-        int32_t sv = __shfl(vstart, cand);
-        if (sv <= ve) r = cand;
-      }
-      const int32_t rs = __shfl(vstart, r);
-      const int64_t rbase = __shfl(rb, r);
-      const uint64_t nr = shfl64(need, r);
-      const uint64_t gr = shfl64(got, r);
-      uint64_t h = 0;
-      int32_t u = 0;
-      if (ve < total && (nr & ~gr)) {
-        u = A.in_idx[rbase + (ve - rs)];
-        h = fr[u] & nr & ~gr;
-        scanned++;
-      }
-      // segmented inclusive prefix-OR; the segment of lane starts at head
-      const int head = rs > base ? rs - base : 0;
-      uint64_t x = h;
+      for (int k = 0; k < kUnroll; k++) {
+        const int32_t ve = base0 + k * kWave + lane;
+        // row r: the largest lane with vstart <= ve (rows of length 0 share
+        // the next row's start, so r always has entries)
+        int r = 0;
 #pragma unroll
-      for (int off = 1; off < kWave; off <<= 1) {
-        uint64_t y = shfl_up64(x, off);
-        if (lane - off >= head) x |= y;
+        for (int step = 32; step > 0; step >>= 1) {
+          int cand = r + step;
+          int32_t sv = __shfl(vstart, cand);
+          if (sv <= ve) r = cand;
+        }
+        rk[k] = r;
+        rsk[k] = __shfl(vstart, r);
+        const int64_t rbase = __shfl(rb, r);
+        nrk[k] = shfl64(need, r);
+        const uint64_t gr = shfl64(got, r);
+        idx[k] = -1;
+        if (ve < total && (nrk[k] & ~gr)) idx[k] = A.in_slot[rbase + (ve - rsk[k])];
       }
-      uint64_t prev = shfl_up64(x, 1);
-      uint64_t mine = h & ~(lane > head ? prev : 0ull);
-      if (mine) ms_emit(A, mine, u, ch * kWave + r, lev);
-      // each row owner collects its row's OR over this step
-      const int32_t my_end = vstart + len;
-      int last = (my_end < base + kWave ? my_end : base + kWave) - 1 - base;
-      bool hit_step = len && vstart < base + kWave && my_end > base;
-      uint64_t tot = shfl64(x, hit_step ? last : lane);
-      if (hit_step) got |= tot;
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) {
+        raw[k] = 0;
+        if (idx[k] >= 0) {
+          raw[k] = fr[idx[k]] & nrk[k];
+          scanned++;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) {
+        const int32_t base = base0 + k * kWave;
+        const int r = rk[k], rs = rsk[k];
+        uint64_t h = raw[k] & ~shfl64(got, r);
+        // segmented inclusive prefix-OR; the segment of lane starts at head
+        const int head = rs > base ? rs - base : 0;
+        uint64_t x = h;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          uint64_t y = shfl_up64(x, off);
+          if (lane - off >= head) x |= y;
+        }
+        uint64_t prev = shfl_up64(x, 1);
+        uint64_t mine = h & ~(lane > head ? prev : 0ull);
+        const uint16_t off = (uint16_t)(base + lane - rs);
+        while (mine) {
+          int l = __ffsll((unsigned long long)mine) - 1;
+          mine &= mine - 1;
+          pt[l][r] = off;
+        }
+        // each row owner collects its row's OR over this step
+        int last = (my_end < base + kWave ? my_end : base + kWave) - 1 - base;
+        bool hit_step = len && vstart < base + kWave && my_end > base;
+        uint64_t tot = shfl64(x, hit_step ? last : lane);
+        if (hit_step) got |= tot;
+      }
       if (!__any(len && (need & ~got))) break;  // every row of the chunk satisfied
     }
-    bool any = got != 0;
-    if (any) {
-      nx[v] = got;
-      A.vis[v] = vw | got;
+    // Flush: per lane bit, 64 consecutive vertices' parents and levels.
+    uint64_t bits = got;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      uint32_t lo = __shfl_xor((uint32_t)bits, off), hi = __shfl_xor((uint32_t)(bits >> 32), off);
+      bits |= ((uint64_t)hi << 32) | lo;
+    }
+    while (bits) {
+      int l = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      if ((got >> l) & 1ull) {
+        int64_t o = (int64_t)l * A.n + v;
+        A.parents[o] = A.in_idx[rb + pt[l][lane]];
+        A.levels[o] = lev;
+      }
+    }
+    if (got) {
+      nx[sv_me] = got;
+      A.vis[sv_me] = vw | got;
       mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
       pairs += (unsigned long long)__popcll(got);
+      nf++;
     }
-    int64_t slot = wave_append(tail, any);
-    if (any) QN[slot] = (int32_t)v;
   }
   // Big vertices: one segment per wavefront.
   for (int64_t s = wave; s < A.pin.nseg; s += nwaves) {
     int32_t v = A.pin.seg_v[s];
-    uint64_t need = A.act & ~A.vis[v];
+    uint64_t need = A.act & ~A.vis[A.slot_of[v]];
     if (!need) continue;
     int32_t bid = A.pin.seg_bid[s];
     int64_t beg = A.pin.seg_beg[s];
     int64_t rend = A.in_ptr[v + 1];
     int64_t end = beg + kSeg < rend ? beg + kSeg : rend;
-    uint64_t gj = ms_wave_pull(A, fr, v, beg, end, need, lev, &A.cand[(int64_t)bid * 64], scanned);
+    uint64_t gj = ms_wave_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], scanned);
     if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
   }
-  unsigned long long s = ms_block_sum(scanned, red);
+  unsigned long long s = ms_block_sum(nf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->nf, s);
+  s = ms_block_sum(scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
   s = ms_block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
@@ -496,16 +611,11 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
 __global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int d) {
   MsCtrl *c = A.ctrl;
   uint64_t *nx = A.fw[(d + 1) & 1];
-  int32_t *QN = A.q[(d + 1) & 1];
-  unsigned long long *tail = &c->qtail[(d + 1) & 1];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t nround = ((A.pin.nbig + stride - 1) / stride) * stride;
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nround; b += stride) {
-    uint64_t got = 0;
-    int32_t v = 0;
-    if (b < A.pin.nbig) {
-      v = A.pin.big_v[b];
-      got = A.bgot[b];
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < A.pin.nbig; b += stride) {
+    {
+      const int32_t v = A.pin.big_v[b];
+      const uint64_t got = A.bgot[b];
       if (got) {
         uint64_t lanes = got;
         while (lanes) {
@@ -517,15 +627,42 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int d) {
           A.cand[b * 64 + l] = 0xFFFFFFFFu;
         }
         A.bgot[b] = 0;
-        nx[v] = got;
-        A.vis[v] |= got;
+        const int32_t sv = A.slot_of[v];
+        nx[sv] = got;
+        A.vis[sv] |= got;
         atomicAdd(&c->mf, (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]));
         atomicAdd(&c->pairs, (unsigned long long)__popcll(got));
+        atomicAdd(&c->nf, 1ull);
       }
     }
-    int64_t slot = wave_append(tail, got != 0);
-    if (got) QN[slot] = v;
   }
+}
+
+// Active list from the frontier words (slot order -> vertex ids), for a push
+// round that follows a pull round (pull rounds keep no list).
+__global__ void __launch_bounds__(kMsBlock) ms_compact(MsArgs A, int d, const int32_t *vertex_of) {
+  MsCtrl *c = A.ctrl;
+  const uint64_t *fr = A.fw[d & 1];
+  __shared__ int32_t lq[kMsLdsQueue];
+  __shared__ unsigned lq_n;
+  __shared__ unsigned long long lq_base;
+  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kMsLdsQueue, &c->qtail[d & 1], A.q[d & 1]};
+  sq.init();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nround = ((A.n + stride - 1) / stride) * stride;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nround; k += stride) {
+    bool on = k < A.n && fr[k] != 0;
+    sq.push(on, on ? vertex_of[k] : 0);
+  }
+  sq.flush();
+}
+
+// Clears a whole frontier-word buffer (after a round whose frontier has no
+// list).
+__global__ void __launch_bounds__(kMsBlock) ms_clear_all(uint64_t *w, int64_t n) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    w[k] = 0;
 }
 
 // Push pass (phase 0: OR into nx and append first writers; phase 1: offer
@@ -538,11 +675,16 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
   const int32_t *Q = A.q[d & 1];
   int32_t *QN = A.q[(d + 1) & 1];
   unsigned long long *tail = &c->qtail[(d + 1) & 1];
-  const int64_t nq = (int64_t)c->qtail[d & 1];
+  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1]);
   const int lane = lane_id();
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t wave = gtid / kWave, nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
   unsigned long long scanned = 0, rows = 0;
+  __shared__ int32_t lq[kMsLdsQueue];
+  __shared__ unsigned lq_n;
+  __shared__ unsigned long long lq_base;
+  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kMsLdsQueue, tail, QN};
+  if (phase == 0) sq.init();
   // Work items: queue entries (skipping big vertices), then big segments of
   // active big vertices.  Both loops are wavefront-uniform.
   const int64_t nitems = nq + A.pout.nseg;
@@ -561,7 +703,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
       int64_t rend = A.out_ptr[u + 1];
       end = beg + kSeg < rend ? beg + kSeg : rend;
     }
-    uint64_t f = fr[u];
+    uint64_t f = fr[A.slot_of[u]];
     if (!f) continue;
     if (lane == 0) {
       scanned += (unsigned long long)(end - beg);
@@ -573,14 +715,15 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
       int32_t w = 0;
       if (e < end) {
         w = A.out_idx[e];
+        const int32_t sw = A.out_slot[e];
         if (phase == 0) {
-          uint64_t nv = f & ~A.vis[w];
+          uint64_t nv = f & ~A.vis[sw];
           if (nv) {
-            uint64_t old = atomicOr((unsigned long long *)&nx[w], (unsigned long long)nv);
+            uint64_t old = atomicOr((unsigned long long *)&nx[sw], (unsigned long long)nv);
             first = (old == 0);
           }
         } else {
-          uint64_t nv = f & nx[w];
+          uint64_t nv = f & nx[sw];
           while (nv) {
             int l = __ffsll((unsigned long long)nv) - 1;
             nv &= nv - 1;
@@ -588,12 +731,10 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
           }
         }
       }
-      if (phase == 0) {
-        int64_t slot = wave_append(tail, first);
-        if (first) QN[slot] = w;
-      }
+      if (phase == 0) sq.push(first, w);
     }
   }
+  if (phase == 0) sq.flush();
   unsigned long long s = ms_block_sum(scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
   s = ms_block_sum(rows, red);
@@ -607,12 +748,13 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int d) {
   __shared__ unsigned long long red[kMsWaves];
   const uint64_t *nx = A.fw[(d + 1) & 1];
   const int32_t *QN = A.q[(d + 1) & 1];
-  const int64_t nq = (int64_t)c->qtail[(d + 1) & 1];
+  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[(d + 1) & 1]);
   unsigned long long mf = 0, pairs = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
        i += (int64_t)gridDim.x * blockDim.x) {
     int32_t v = QN[i];
-    uint64_t nw = nx[v];
+    const int32_t sv = A.slot_of[v];
+    uint64_t nw = nx[sv];
     uint64_t lanes = nw;
     while (lanes) {
       int l = __ffsll((unsigned long long)lanes) - 1;
@@ -621,7 +763,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int d) {
       A.parents[o] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
       A.levels[o] = (int16_t)(d + 1);
     }
-    A.vis[v] |= nw;
+    A.vis[sv] |= nw;
     mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
     pairs += (unsigned long long)__popcll(nw);
   }
@@ -637,50 +779,57 @@ __global__ void __launch_bounds__(kMsBlock) ms_clear(MsArgs A, int d) {
   MsCtrl *c = A.ctrl;
   uint64_t *fr = A.fw[d & 1];
   const int32_t *Q = A.q[d & 1];
-  const int64_t nq = (int64_t)c->qtail[d & 1];
+  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1]);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
        i += (int64_t)gridDim.x * blockDim.x)
-    fr[Q[i]] = 0;
+    fr[A.slot_of[Q[i]]] = 0;
 }
 
-__global__ void ms_decide(MsArgs A, int d) {
+__global__ void ms_decide(MsArgs A, int d, MsParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   MsCtrl *c = A.ctrl;
-  const long long nf = (long long)c->qtail[(d + 1) & 1];
+  const int cur = ld_ctrl(&c->dir);
+  // push rounds list the new active vertices; pull rounds only count them
+  const long long nf =
+      (long long)(cur == 0 ? ld_ctrl(&c->qtail[(d + 1) & 1]) : ld_ctrl(&c->nf));
+  const long long mf = (long long)ld_ctrl(&c->mf);
   if (d < kMsHist) {
     A.hist->nf[d] = nf;
-    A.hist->scanned[d] = (long long)c->scanned;
-    A.hist->rows[d] = (long long)c->rows;
-    A.hist->pairs[d] = (long long)c->pairs;
-    A.hist->dir[d] = c->dir;
+    A.hist->scanned[d] = (long long)ld_ctrl(&c->scanned);
+    A.hist->rows[d] = (long long)ld_ctrl(&c->rows);
+    A.hist->pairs[d] = (long long)ld_ctrl(&c->pairs);
+    A.hist->dir[d] = cur;
   }
-  c->rounds = d + 1;
-  c->qtail[d & 1] = 0;
-  const long long mf = (long long)c->mf;
-  c->mf = c->scanned = c->rows = c->pairs = 0;
+  st_ctrl(&c->nf, 0ull);
+  st_ctrl(&c->mf, 0ull);
+  st_ctrl(&c->scanned, 0ull);
+  st_ctrl(&c->rows, 0ull);
+  st_ctrl(&c->pairs, 0ull);
+  st_ctrl(&c->qtail[d & 1], 0ull);
+  st_ctrl(&c->rounds, (long long)(d + 1));
   if (nf == 0) {
-    c->done = 1;
+    st_ctrl(&c->done, 1);
     return;
   }
-  int dir = c->dir;
-  switch (c->mode) {
+  int dir = cur;
+  switch (P.mode) {
     case CGBFS_DIR_TOPDOWN: dir = 0; break;
     case CGBFS_DIR_BOTTOMUP: dir = 1; break;
     case CGBFS_DIR_TAU:
-      if (dir == 0 && nf > c->tau) dir = 1;
+      if (dir == 0 && nf > P.tau) dir = 1;
       break;
     default:
       if (dir == 0) {
-        if ((double)mf * c->alpha > (double)c->nnz) dir = 1;
+        if ((double)mf * P.alpha > (double)P.nnz) dir = 1;
       } else {
-        if ((double)nf * c->beta < (double)A.n && nf < c->prev_nf) dir = 0;
+        if ((double)nf * P.beta < (double)A.n && nf < ld_ctrl(&c->prev_nf)) dir = 0;
       }
   }
-  c->prev_nf = nf;
-  c->dir = dir;
-  if (d + 1 >= c->max_levels) {
-    c->status = CGBFS_ETIMEOUT;
-    c->done = 1;
+  st_ctrl(&c->prev_nf, nf);
+  st_ctrl(&c->dir, dir);
+  if (d + 1 >= P.max_levels) {
+    st_ctrl(&c->status, (int)CGBFS_ETIMEOUT);
+    st_ctrl(&c->done, 1);
   }
 }
 
@@ -688,7 +837,7 @@ __global__ void ms_decide(MsArgs A, int d) {
 __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < A.n;
        v += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t miss = A.act & ~A.vis[v];
+    uint64_t miss = A.act & ~A.vis[A.slot_of[v]];
     while (miss) {
       int l = __ffsll((unsigned long long)miss) - 1;
       miss &= miss - 1;
@@ -700,17 +849,17 @@ __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
 }
 
 // GTEPS m per lane: sum of the per-vertex weight over each lane's reached set.
-__global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const int32_t *w,
-                                                     int64_t n, int nroots,
+__global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const int32_t *slot_of,
+                                                     const int32_t *w, int64_t n, int nroots,
                                                      unsigned long long *out) {
   __shared__ unsigned long long acc[64];
   if (threadIdx.x < 64) acc[threadIdx.x] = 0;
   __syncthreads();
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t b = vis[v];
     unsigned long long wv = (unsigned long long)w[v];
     if (!wv) continue;
+    uint64_t b = vis[slot_of[v]];
     while (b) {
       int l = __ffsll((unsigned long long)b) - 1;
       b &= b - 1;
@@ -775,6 +924,9 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   A.hist = ws.hist;
   A.pin = md->pin;
   A.pout = md->pout;
+  A.slot_of = md->slot_of;
+  A.in_slot = md->in_slot;
+  A.out_slot = md->out_slot;
   A.cand = ws.cand;
   A.bgot = ws.bgot;
   A.n = g->n;
@@ -802,8 +954,14 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
     return hip_fail(e, "msbfs setup", __FILE__, __LINE__);
   }
   ms_init_fill<<<blocks, kMsBlock, 0, st>>>(A);
-  ms_init_roots<<<1, 64, 0, st>>>(A, ws.roots, o.direction, o.alpha, o.beta, (long long)o.tau,
-                                  (long long)o.max_levels, (long long)g->nnz);
+  MsParams P;
+  P.mode = o.direction;
+  P.alpha = o.alpha;
+  P.beta = o.beta;
+  P.tau = (long long)o.tau;
+  P.max_levels = (long long)o.max_levels;
+  P.nnz = (long long)g->nnz;
+  ms_init_roots<<<1, 64, 0, st>>>(A, ws.roots, P);
   MsCtrl hc;
   e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -812,6 +970,10 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   std::vector<std::pair<hipEvent_t, hipEvent_t>> rev;
   std::vector<int> rdir;
   int64_t d = 0;
+  // listed: the current frontier's vertices are listed in Q[d&1] (the roots,
+  // or the output of a push round).  A pull round leaves no list; a push
+  // round after it first compacts one from the frontier words.
+  bool listed = true;
   while (e == hipSuccess && !hc.done) {
     hipEvent_t ra, rb;
     if ((e = new_event(&ra)) != hipSuccess || (e = new_event(&rb)) != hipSuccess) break;
@@ -820,15 +982,19 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
       ms_pull<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
       if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, (int)d);
     } else {
+      if (!listed) ms_compact<<<blocks, kMsBlock, 0, st>>>(A, (int)d, md->vertex_of);
       ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 0);
       ms_prep<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
       ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 1);
+      listed = true;
     }
     hipEventRecord(rb, st);
     rev.push_back({ra, rb});
     rdir.push_back(hc.dir);
-    ms_clear<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
-    ms_decide<<<1, 64, 0, st>>>(A, (int)d);
+    if (listed) ms_clear<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
+    else ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A.fw[d & 1], g->n);
+    listed = (hc.dir == 0);  // the next frontier is listed iff this round pushed
+    ms_decide<<<1, 64, 0, st>>>(A, (int)d, P);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -901,7 +1067,8 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (m_lanes) {
     unsigned long long m[64];
     ms_zero<<<1, 64, 0, st>>>(ws.mlane, 64);
-    ms_count<<<grid_blocks(g->device, 2), kMsBlock, 0, st>>>(ws.vis, g->m_weight, g->n, nroots, ws.mlane);
+    ms_count<<<grid_blocks(g->device, 2), kMsBlock, 0, st>>>(ws.vis, md->slot_of, g->m_weight, g->n,
+                                                             nroots, ws.mlane);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(m, ws.mlane, sizeof(m), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

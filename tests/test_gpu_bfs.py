@@ -100,6 +100,14 @@ def test_edge_cases():
   cases.append((_edges_graph([(0, i) for i in range(1, 20001)] + [(20000, 20001)], 20003,
                              directed=False), [0, 7, 20001, 20002]))
   cases.append((_edges_graph([(i, (i * 7 + 3) % 777) for i in range(777)], 777), [0, 5, 776]))
+  # 5000 frontier vertices of degree >= 300 in one push round: the flat
+  # whole-grid expansion stages its degree prefix in passes of 4096
+  rng = np.random.default_rng(5)
+  hubs, pool = 5000, 20000
+  e = [(0, h) for h in range(1, hubs + 1)]
+  leaves = rng.integers(hubs + 1, hubs + 1 + pool, size=(hubs, 300))
+  e += [(h, int(x)) for h in range(1, hubs + 1) for x in leaves[h - 1]]
+  cases.append((_edges_graph(e, hubs + 1 + pool, directed=False), [0, 17]))
   for g, roots in cases:
     with _device(g) as dg:
       for s in roots:
