@@ -43,9 +43,10 @@ namespace cg {
 constexpr int kMsBlock = 256;
 constexpr int kMsWaves = kMsBlock / kWave;
 constexpr int kMsHist = 4096;
-constexpr int64_t kSeg = 1024;      // segment length and wavefront-tier cap
+constexpr int64_t kSeg = 512;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 4;          // pull steps whose loads are issued together
+constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
 
 int grid_blocks(int device, int mult);
 void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o);
@@ -566,13 +567,26 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
       uint32_t lo = __shfl_xor((uint32_t)bits, off), hi = __shfl_xor((uint32_t)(bits >> 32), off);
       bits |= ((uint64_t)hi << 32) | lo;
     }
+    // kFlush bits at a time: their parent-id loads are in flight together
+    // before any store is issued.
     while (bits) {
-      int l = __ffsll((unsigned long long)bits) - 1;
-      bits &= bits - 1;
-      if ((got >> l) & 1ull) {
-        int64_t o = (int64_t)l * A.n + v;
-        A.parents[o] = A.in_idx[rb + pt[l][lane]];
-        A.levels[o] = lev;
+      int lb[kFlush];
+      int32_t pv[kFlush];
+#pragma unroll
+      for (int k = 0; k < kFlush; k++) {
+        lb[k] = bits ? __ffsll((unsigned long long)bits) - 1 : -1;
+        bits &= bits - 1;
+      }
+#pragma unroll
+      for (int k = 0; k < kFlush; k++)
+        pv[k] = (lb[k] >= 0 && ((got >> lb[k]) & 1ull)) ? A.in_idx[rb + pt[lb[k]][lane]] : 0;
+#pragma unroll
+      for (int k = 0; k < kFlush; k++) {
+        if (lb[k] >= 0 && ((got >> lb[k]) & 1ull)) {
+          int64_t o = (int64_t)lb[k] * A.n + v;
+          A.parents[o] = pv[k];
+          A.levels[o] = lev;
+        }
       }
     }
     if (got) {
