@@ -37,12 +37,12 @@ def harmonic_mean(values):
 
 
 def pull_round_bytes(n, rows, edges, pairs, nf):
-  """Algorithmic HBM bytes of one multi-source pull round: lane words of
-  every vertex (8 V), offsets of the adjacency lists opened (8 U), 4 B column
-  id + 8 B frontier word per adjacency entry read (12 E), 4 B parent + 2 B
-  level per reached (lane, vertex) pair (6 P), and 8 B vis + 8 B next word +
-  4 B queue entry per newly active vertex (20 N)."""
-  return 8 * n + 8 * rows + 12 * edges + 6 * pairs + 20 * nf
+  """Algorithmic HBM bytes of the pull part of one multi-source round: lane
+  words of every vertex (8 V), offsets of the adjacency lists opened (8 U),
+  4 B column id + 8 B frontier word per adjacency entry read (12 E), 4 B
+  parent + 2 B level per reached (lane, vertex) pair (6 P), and 8 B vis +
+  8 B next word per newly active vertex (16 N)."""
+  return 8 * n + 8 * rows + 12 * edges + 6 * pairs + 16 * nf
 
 
 def parse_args():
@@ -117,13 +117,19 @@ def main():
   batched_aggregate = m.sum() / t_step / 1e9
   lane_hmean = harmonic_mean(m / t_step / 1e9)
 
-  # Roofline of the dominant kernel: the multi-source pull round.
+  # Roofline of the dominant kernel: the pull part of the multi-source
+  # rounds (ms_pull + ms_pull_fin), timed by the library's HIP events.
   st = last.stats
-  pull = st.direction_history == 1
-  byts = sum(pull_round_bytes(n, st.rows_scanned[i], st.edges_scanned[i], st.pairs_found[i],
-                              st.nf_history[i]) for i in np.nonzero(pull)[0])
-  tpull = float(st.level_ms[pull].sum()) / 1e3
-  npull = int(pull.sum())
+  for i in range(int(st.rounds_completed)):
+    log(f"  round {i}: dir {st.direction_history[i]} pull lanes {st.pull_lanes[i]:2d} "
+        f"pull {st.pull_ms[i]:.3f} ms ({st.pull_edges[i]} entries) | push "
+        f"{st.level_ms[i] - st.pull_ms[i]:.3f} ms ({st.edges_scanned[i] - st.pull_edges[i]} entries)"
+        f" | nf {st.nf_history[i]} pairs {st.pairs_found[i]}")
+  pr = np.nonzero(st.pull_lanes > 0)[0]
+  byts = sum(pull_round_bytes(n, st.pull_rows[i], st.pull_edges[i], st.pull_pairs[i],
+                              st.pull_nf[i]) for i in pr)
+  tpull = float(st.pull_ms[pr].sum()) / 1e3
+  npull = len(pr)
   achieved = byts / tpull / 1e9 if tpull > 0 else 0.0
   traffic = None
   prof = os.path.join(HERE, "profiles", "msbfs_pull_s24_traffic.json")

@@ -21,8 +21,9 @@ DIR_BEAMER = 0
 DIR_TOPDOWN = 1
 DIR_BOTTOMUP = 2
 DIR_TAU = 3
+DIR_BEAMER_LANES = 4
 DIRECTIONS = {"beamer": DIR_BEAMER, "topdown": DIR_TOPDOWN, "bottomup": DIR_BOTTOMUP,
-              "tau": DIR_TAU}
+              "tau": DIR_TAU, "beamer_lanes": DIR_BEAMER_LANES}
 
 # Every symbol include/cgbfs.h declares.
 EXPORTED = (
@@ -65,6 +66,12 @@ class Stats(ctypes.Structure):
       ("output_ms", ctypes.c_double),
       ("visited_count", ctypes.c_int64),
       ("m_edges", ctypes.c_int64),
+      ("pull_lanes", ctypes.POINTER(ctypes.c_int64)),
+      ("pull_nf", ctypes.POINTER(ctypes.c_int64)),
+      ("pull_rows", ctypes.POINTER(ctypes.c_int64)),
+      ("pull_edges", ctypes.POINTER(ctypes.c_int64)),
+      ("pull_pairs", ctypes.POINTER(ctypes.c_int64)),
+      ("pull_ms", ctypes.POINTER(ctypes.c_double)),
   ]
 
 

@@ -628,7 +628,9 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
   cgbfs_options o;
   fill_options(opt, g->n, &o);
-  if (o.direction < 0 || o.direction > 3) return set_error(CGBFS_EINVAL, "unknown direction mode");
+  if (o.direction < 0 || o.direction > CGBFS_DIR_BEAMER_LANES)
+    return set_error(CGBFS_EINVAL, "unknown direction mode");
+  if (o.direction == CGBFS_DIR_BEAMER_LANES) o.direction = CGBFS_DIR_BEAMER;  // one lane
   CG_HIP(hipSetDevice(g->device));
   Workspace ws;
   int rc = ensure_ws1(g, &ws);

@@ -17,16 +17,24 @@
 //                       item, and segments of one vertex combine per lane
 //                       with atomicMin / atomicOr in a small scratch area.
 //
-// Pull round (ms_pull + ms_pull_fin): every vertex with an unreached lane
-// ORs the frontier words of its ascending in-neighbours; within a step of
-// 64 neighbours an exclusive prefix-OR across the wavefront gives each lane
-// of each vertex its first (lowest-id) hit.  Stops once every needed lane is
-// found.
-// Push round (ms_push x2 + ms_prep): pass 1 ORs frontier words into nx[w]
-// of out-neighbours (masked by the level-start vis[w]); the first writer of
-// a vertex appends it to the next queue.  ms_prep then marks the new
-// (lane, vertex) pairs, and pass 2 offers every frontier predecessor with a
-// per-lane atomicMin, so the parent is again the lowest id.
+// Directions are chosen per lane: each lane runs the single-root rule on its
+// own counts (its frontier size and degree sums, tallied per lane bit in
+// the kernels), and a round first pulls the lanes in pull mode, then pushes
+// the lanes in push mode.  Roots progress at different speeds (a root next
+// to a hub is a level ahead of a leaf root), so one direction for all lanes
+// would make the pull rescan whole adjacency lists for lanes that are not
+// there yet.  Lanes with an empty frontier take no part.
+// Pull part (ms_pull + ms_pull_fin): every vertex with an unreached pull
+// lane ORs the frontier words of its ascending in-neighbours; within a step
+// of 64 neighbours an exclusive prefix-OR across the wavefront gives each
+// lane of each vertex its first (lowest-id) hit.  Stops once every needed
+// lane is found.
+// Push part (ms_push x2 + ms_prep): pass 1 ORs the push lanes of frontier
+// words into nx[w] of out-neighbours (masked by the level-start vis[w]); the
+// first writer of push bits of a vertex appends it to the next queue.
+// ms_prep then marks the new (lane, vertex) pairs, and pass 2 offers every
+// frontier predecessor with a per-lane atomicMin, so the parent is again
+// the lowest id.
 // Outputs parents[l][v] (int32) and levels[l][v] (int16) are written once
 // per (lane, vertex): at discovery, or by the final pass for unreached
 // pairs.  No initial fill of the 6 B x 64 x V output arrays is needed.
@@ -34,6 +42,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "cg_common.h"
@@ -43,27 +52,48 @@ namespace cg {
 constexpr int kMsBlock = 256;
 constexpr int kMsWaves = kMsBlock / kWave;
 constexpr int kMsHist = 4096;
-constexpr int64_t kSeg = 512;       // segment length and wavefront-tier cap
+constexpr int64_t kSeg = 256;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
-constexpr int kUnroll = 4;          // pull steps whose loads are issued together
-constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
+constexpr int kUnroll = 2;          // pull steps whose loads are issued together
+constexpr int kFlush = 4;           // lane bits per batch of the pull's output flush
+// First-hit offsets within a wavefront-tier row (< kSeg) fit in a byte.
+using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
 int grid_blocks(int device, int mult);
 void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o);
 
+// Round counters come in two parts: the push part (nf .. pairs) and the
+// pull part (pnf .. ppairs).  nf counts only vertices the pull did not
+// reach, so nf + pnf is the number of vertices with a new lane.
 struct MsCtrl {
   unsigned long long qtail[2];
-  unsigned long long nf;       // vertices newly active after a pull round
-  unsigned long long mf;       // out-degree sum of the next active set
+  unsigned long long nf;
   unsigned long long scanned;  // adjacency entries read this round
   unsigned long long rows;     // adjacency lists opened this round
   unsigned long long pairs;    // (lane, vertex) pairs reached this round
+  unsigned long long pnf, pscanned, prows, ppairs;
+  unsigned long long mf;       // out-degree sum of the vertices with a new lane
+  unsigned long long live;     // lanes that reached a vertex this round
+  unsigned long long segtail[2];  // active segments this round: pull, push
+  unsigned long long pmask;    // lanes the next round pulls
+  unsigned long long qmask;    // lanes the next round pushes
   long long prev_nf;
   long long rounds;
-  int dir;
+  int gdir;                    // union-count policy: 0 push, 1 pull
   int done;
   int status;
 };
+
+// Per-lane direction state.  cnt/out/in are this round's tallies of the
+// vertices each lane newly reached (count, out-degree sum, in-degree sum,
+// degrees clamped at kDegClamp); mu is the in-degree sum of the lane's
+// unreached vertices, prev the lane's previous count, dir 0 push / 1 pull.
+struct MsLane {
+  unsigned long long cnt[64], out[64], in[64];
+  long long mu[64], prev[64];
+  int dir[64];
+};
+constexpr uint32_t kDegClamp = 1u << 25;  // 64 clamped degrees sum below 2^32
 
 // Per-search constants, passed by value to the kernels that need them.
 struct MsParams {
@@ -77,7 +107,9 @@ struct MsHist {
   long long scanned[kMsHist];
   long long rows[kMsHist];
   long long pairs[kMsHist];
-  int dir[kMsHist];
+  long long pnf[kMsHist], pscanned[kMsHist], prows[kMsHist], ppairs[kMsHist];
+  long long plane[kMsHist];  // lanes pulled
+  int dir[kMsHist];          // 0 push only, 1 pull only, 2 both
 };
 
 // Static segment list of one adjacency view (per graph).
@@ -88,6 +120,7 @@ struct SegList {
   int32_t *seg_bid = nullptr;  // big-vertex index of each segment
   int64_t *seg_beg = nullptr;  // first adjacency entry of each segment
   int32_t *big_v = nullptr;    // big vertices, by index
+  int32_t *seg_first = nullptr; // [nbig + 1] first segment of each big vertex
   uint8_t *is_big = nullptr;   // per vertex
 };
 
@@ -100,11 +133,18 @@ struct MsArgs {
   int16_t *levels;   // [nroots][n]
   uint64_t *vis;
   uint64_t *fw[2];   // frontier words, rotated by level parity
+  // One bit per slot: the current frontier word is nonzero.  Rebuilt before
+  // each pull part (ms_mark); the pull reads a frontier word only where the
+  // bit is set, so most of its gathers hit this small, cached array.
+  uint32_t *fb;
   int32_t *q[2];
   MsCtrl *ctrl;
   MsHist *hist;
-  // big-vertex segments, pull (in view) and push (out view)
+  MsLane *lanes;
+  // big-vertex segments, pull (in view) and push (out view), and this
+  // round's active ones (indices into the static lists)
   SegList pin, pout;
+  int32_t *segq[2];
   // Vertex-state slot of each vertex (hub-first order) and the adjacency
   // arrays re-expressed in slots, entry for entry; the lane words vis/fw
   // are indexed by slot, everything else by vertex id.
@@ -116,6 +156,7 @@ struct MsArgs {
   int64_t n;
   uint64_t act;      // mask of lanes in use
   int nroots;
+  int symmetric;     // in view == out view
 };
 
 static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -131,13 +172,16 @@ struct MsGraphData {
 
 struct MsWorkspace {
   uint64_t *vis, *fw[2];
+  uint32_t *fb;
   int32_t *q[2];
   MsCtrl *ctrl;
   MsHist *hist;
+  MsLane *lanes;
   unsigned long long *mlane;
   int64_t *roots;
   uint32_t *cand;
   uint64_t *bgot;
+  int32_t *segq[2];
 };
 
 // ---------------------------------------------------------------------------
@@ -145,13 +189,14 @@ struct MsWorkspace {
 // ---------------------------------------------------------------------------
 static void seg_plan(const std::vector<int64_t> &ptr, int64_t n, std::vector<int32_t> &sv,
                      std::vector<int32_t> &sb, std::vector<int64_t> &sg, std::vector<int32_t> &bv,
-                     std::vector<uint8_t> &isb) {
+                     std::vector<int32_t> &bf, std::vector<uint8_t> &isb) {
   isb.assign(n, 0);
   for (int64_t v = 0; v < n; v++) {
     int64_t deg = ptr[v + 1] - ptr[v];
     if (deg < kSeg) continue;
     int32_t bid = (int32_t)bv.size();
     bv.push_back((int32_t)v);
+    bf.push_back((int32_t)sv.size());
     isb[v] = 1;
     for (int64_t b = ptr[v]; b < ptr[v + 1]; b += kSeg) {
       sv.push_back((int32_t)v);
@@ -159,27 +204,28 @@ static void seg_plan(const std::vector<int64_t> &ptr, int64_t n, std::vector<int
       sg.push_back(b);
     }
   }
+  bf.push_back((int32_t)sv.size());
 }
 
 static int build_seglists(DeviceGraph *g, MsGraphData *md) {
   const int64_t n = g->n;
   std::vector<int64_t> ptr_in(n + 1), ptr_out;
   CG_HIP(hipMemcpy(ptr_in.data(), g->in_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
-  std::vector<int32_t> sv[2], sb[2], bv[2];
+  std::vector<int32_t> sv[2], sb[2], bv[2], bf[2];
   std::vector<int64_t> sg[2];
   std::vector<uint8_t> isb[2];
-  seg_plan(ptr_in, n, sv[0], sb[0], sg[0], bv[0], isb[0]);
+  seg_plan(ptr_in, n, sv[0], sb[0], sg[0], bv[0], bf[0], isb[0]);
   const bool shared = g->in_ptr == g->out_ptr;
   if (!shared) {
     ptr_out.resize(n + 1);
     CG_HIP(hipMemcpy(ptr_out.data(), g->out_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
-    seg_plan(ptr_out, n, sv[1], sb[1], sg[1], bv[1], isb[1]);
+    seg_plan(ptr_out, n, sv[1], sb[1], sg[1], bv[1], bf[1], isb[1]);
   }
   const int views = shared ? 1 : 2;
   size_t total = 0;
   for (int k = 0; k < views; k++)
     total += al(sv[k].size() * 4 + 4) + al(sb[k].size() * 4 + 4) + al(sg[k].size() * 8 + 8) +
-             al(bv[k].size() * 4 + 4) + al((size_t)n);
+             al(bv[k].size() * 4 + 4) + al(bf[k].size() * 4) + al((size_t)n);
   CG_HIP(hipMalloc(&md->mem, total));
   char *p = (char *)md->mem;
   for (int k = 0; k < views; k++) {
@@ -190,6 +236,7 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
     L.seg_bid = (int32_t *)p; p += al(sb[k].size() * 4 + 4);
     L.seg_beg = (int64_t *)p; p += al(sg[k].size() * 8 + 8);
     L.big_v = (int32_t *)p; p += al(bv[k].size() * 4 + 4);
+    L.seg_first = (int32_t *)p; p += al(bf[k].size() * 4);
     L.is_big = (uint8_t *)p; p += al((size_t)n);
     if (L.nseg) {
       CG_HIP(hipMemcpy(L.seg_v, sv[k].data(), sv[k].size() * 4, hipMemcpyHostToDevice));
@@ -197,6 +244,7 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
       CG_HIP(hipMemcpy(L.seg_beg, sg[k].data(), sg[k].size() * 8, hipMemcpyHostToDevice));
       CG_HIP(hipMemcpy(L.big_v, bv[k].data(), bv[k].size() * 4, hipMemcpyHostToDevice));
     }
+    CG_HIP(hipMemcpy(L.seg_first, bf[k].data(), bf[k].size() * 4, hipMemcpyHostToDevice));
     CG_HIP(hipMemcpy(L.is_big, isb[k].data(), (size_t)n, hipMemcpyHostToDevice));
   }
   if (shared) md->pout = md->pin;
@@ -298,9 +346,11 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
   *mdp = md;
   const int64_t nbig = md->pin.nbig;
   size_t wb = al(sizeof(uint64_t) * n), qb = al(sizeof(int32_t) * n);
-  size_t total = 3 * wb + 2 * qb + al(sizeof(MsCtrl)) + al(sizeof(MsHist)) +
+  size_t fbb = al(sizeof(uint64_t) * ((n + 63) / 64));  // whole 64-bit words per wavefront
+  size_t total = 3 * wb + fbb + 2 * qb + al(sizeof(MsCtrl)) + al(sizeof(MsHist)) + al(sizeof(MsLane)) +
                  al(sizeof(unsigned long long) * 64) + al(sizeof(int64_t) * 64) +
-                 al(sizeof(uint32_t) * 64 * (nbig + 1)) + al(sizeof(uint64_t) * (nbig + 1));
+                 al(sizeof(uint32_t) * 64 * (nbig + 1)) + al(sizeof(uint64_t) * (nbig + 1)) +
+                 al(sizeof(int32_t) * (md->pin.nseg + 1)) + al(sizeof(int32_t) * (md->pout.nseg + 1));
   if (g->wsm_bytes < total) {
     if (g->wsm) hipFree(g->wsm);
     g->wsm = nullptr;
@@ -312,14 +362,18 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
   w->vis = (uint64_t *)p; p += wb;
   w->fw[0] = (uint64_t *)p; p += wb;
   w->fw[1] = (uint64_t *)p; p += wb;
+  w->fb = (uint32_t *)p; p += fbb;
   w->q[0] = (int32_t *)p; p += qb;
   w->q[1] = (int32_t *)p; p += qb;
   w->ctrl = (MsCtrl *)p; p += al(sizeof(MsCtrl));
   w->hist = (MsHist *)p; p += al(sizeof(MsHist));
+  w->lanes = (MsLane *)p; p += al(sizeof(MsLane));
   w->mlane = (unsigned long long *)p; p += al(sizeof(unsigned long long) * 64);
   w->roots = (int64_t *)p; p += al(sizeof(int64_t) * 64);
   w->cand = (uint32_t *)p; p += al(sizeof(uint32_t) * 64 * (nbig + 1));
-  w->bgot = (uint64_t *)p;
+  w->bgot = (uint64_t *)p; p += al(sizeof(uint64_t) * (nbig + 1));
+  w->segq[0] = (int32_t *)p; p += al(sizeof(int32_t) * (md->pin.nseg + 1));
+  w->segq[1] = (int32_t *)p;
   return CGBFS_OK;
 }
 
@@ -360,6 +414,88 @@ __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
   return s;
 }
 
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t lo = __shfl_xor((uint32_t)x, off), hi = __shfl_xor((uint32_t)(x >> 32), off);
+    x |= ((uint64_t)hi << 32) | lo;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint32_t clamp_deg(int64_t d) {
+  return d < (int64_t)kDegClamp ? (uint32_t)d : kDegClamp;
+}
+
+// Per-lane tallies of newly reached vertices.  Lane l of a wavefront holds
+// the totals for lane bit l; tally_flush() folds a block's wavefronts into
+// MsLane with one global atomic per lane bit and field.
+struct LaneTally {
+  unsigned long long cnt = 0, out = 0, in = 0;
+};
+
+// Wavefront-uniform: `word` is this lane's vertex's new lane bits (0 for
+// none), od/id its clamped out-/in-degree.
+__device__ __forceinline__ void tally_words(uint64_t word, uint32_t od, uint32_t id, bool sym,
+                                            LaneTally &t) {
+  uint64_t bits = wave_or64(word);
+  const int lane = lane_id();
+  while (bits) {
+    const int l = __ffsll((unsigned long long)bits) - 1;
+    bits &= bits - 1;
+    const bool hit = (word >> l) & 1ull;
+    const unsigned c = (unsigned)__popcll(__ballot(hit));
+    const uint32_t so = wave_sum_u32(hit ? od : 0u);
+    const uint32_t si = sym ? so : wave_sum_u32(hit ? id : 0u);
+    if (lane == l) {
+      t.cnt += c;
+      t.out += so;
+      t.in += si;
+    }
+  }
+}
+
+// lds: 192 entries, zeroed (and synchronised) by the caller beforehand.
+// Every thread of the block must call it.
+__device__ __forceinline__ void tally_flush(const LaneTally &t, MsLane *L,
+                                            unsigned long long *lds) {
+  const int lane = lane_id();
+  if (t.cnt) {
+    atomicAdd(&lds[lane], t.cnt);
+    atomicAdd(&lds[64 + lane], t.out);
+    atomicAdd(&lds[128 + lane], t.in);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && lds[threadIdx.x]) {
+    atomicAdd(&L->cnt[threadIdx.x], lds[threadIdx.x]);
+    atomicAdd(&L->out[threadIdx.x], lds[64 + threadIdx.x]);
+    atomicAdd(&L->in[threadIdx.x], lds[128 + threadIdx.x]);
+  }
+}
+
+// ORs every thread's lane mask into *dst with one global atomic per block.
+// Every thread of the block must call it.
+__device__ __forceinline__ void live_flush(uint64_t v, unsigned long long *dst) {
+  __shared__ unsigned long long blk;
+  v = wave_or64(v);
+  if (threadIdx.x == 0) blk = 0;
+  __syncthreads();
+  if (lane_id() == 0 && v) atomicOr(&blk, (unsigned long long)v);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk) atomicOr(dst, blk);
+}
+
+__device__ __forceinline__ void tally_lds_init(unsigned long long *lds) {
+  if (threadIdx.x < 192) lds[threadIdx.x] = 0;
+  __syncthreads();
+}
+
 
 // Wavefront scan of in-neighbour entries [beg, end) for lanes `need`
 // (wave-uniform arguments): gathers lane words through in_slot and offers
@@ -367,13 +503,18 @@ __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
 // atomicMin.  Returns the lanes found (wave-uniform).
 __device__ __forceinline__ uint64_t ms_wave_pull(const MsArgs &A, const uint64_t *fr, int64_t beg,
                                                  int64_t end, uint64_t need, uint32_t *cand,
-                                                 unsigned long long &scanned) {
+                                                 bool use_fb, uint32_t &scanned) {
   const int lane = lane_id();
   uint64_t got = 0;
   for (int64_t e0 = beg; e0 < end; e0 += kWave) {
     int64_t e = e0 + lane;
     uint64_t h = 0;
-    if (e < end) h = fr[A.in_slot[e]] & need & ~got;
+    if (e < end) {
+      const int32_t s = A.in_slot[e];
+      if (!use_fb || ((A.fb[s >> 5] >> (s & 31)) & 1u)) h = fr[s] & need & ~got;
+    }
+    if (lane == 0) scanned += (uint32_t)((end - e0) < kWave ? (end - e0) : kWave);
+    if (!__any(h != 0)) continue;  // no frontier in-neighbour in this step
     uint64_t inc = wave_scan_or(h);
     uint64_t before = shfl_up64(inc, 1);
     if (lane == 0) before = 0;
@@ -387,7 +528,6 @@ __device__ __forceinline__ uint64_t ms_wave_pull(const MsArgs &A, const uint64_t
       }
     }
     got |= shfl64(inc, kWave - 1);
-    if (lane == 0) scanned += (unsigned long long)((end - e0) < kWave ? (end - e0) : kWave);
     if (got == need) break;
   }
   return got;
@@ -408,60 +548,121 @@ __global__ void __launch_bounds__(kMsBlock) ms_init_fill(MsArgs A) {
   for (int64_t i = tid; i < A.pin.nbig; i += stride) A.bgot[i] = 0;
 }
 
+// Whether the direction policy needs the per-lane tallies.
+__host__ __device__ inline bool lane_policy(int mode) {
+  return mode == CGBFS_DIR_TAU || mode == CGBFS_DIR_BEAMER_LANES;
+}
+
+// One wavefront: thread 0 seeds the roots; thread l sets lane l's direction
+// state the way the single-root search starts (round 0 pushes unless every
+// round pulls, bool_pe.csl:714-719).
 __global__ void ms_init_roots(MsArgs A, const int64_t *roots, MsParams P) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  unsigned long long nq = 0, mf = 0;
-  for (int l = 0; l < A.nroots; l++) {
-    int64_t r = roots[l];
-    int32_t sr = A.slot_of[r];
-    uint64_t bit = 1ull << l;
-    if (A.fw[0][sr] == 0) {
-      A.q[0][nq++] = (int32_t)r;
-      mf += (unsigned long long)(A.out_ptr[r + 1] - A.out_ptr[r]);
-    }
-    A.vis[sr] |= bit;
-    A.fw[0][sr] |= bit;
-    A.parents[(int64_t)l * A.n + r] = (int32_t)r;
-    A.levels[(int64_t)l * A.n + r] = 0;
-  }
+  if (blockIdx.x != 0) return;
   MsCtrl *c = A.ctrl;
-  st_ctrl(&c->qtail[0], nq);
-  st_ctrl(&c->qtail[1], 0ull);
-  st_ctrl(&c->nf, 0ull);
-  st_ctrl(&c->mf, 0ull);
-  st_ctrl(&c->scanned, 0ull);
-  st_ctrl(&c->rows, 0ull);
-  st_ctrl(&c->pairs, 0ull);
-  st_ctrl(&c->prev_nf, (long long)nq);
-  st_ctrl(&c->rounds, 0ll);
+  MsLane *L = A.lanes;
+  const int l = threadIdx.x;
+  if (l == 0) {
+    unsigned long long nq = 0;
+    for (int k = 0; k < A.nroots; k++) {
+      int64_t r = roots[k];
+      int32_t sr = A.slot_of[r];
+      uint64_t bit = 1ull << k;
+      if (A.fw[0][sr] == 0) A.q[0][nq++] = (int32_t)r;
+      A.vis[sr] |= bit;
+      A.fw[0][sr] |= bit;
+      A.parents[(int64_t)k * A.n + r] = (int32_t)r;
+      A.levels[(int64_t)k * A.n + r] = 0;
+    }
+    st_ctrl(&c->qtail[0], nq);
+    st_ctrl(&c->qtail[1], 0ull);
+    st_ctrl(&c->nf, 0ull);
+    st_ctrl(&c->scanned, 0ull);
+    st_ctrl(&c->rows, 0ull);
+    st_ctrl(&c->pairs, 0ull);
+    st_ctrl(&c->pnf, 0ull);
+    st_ctrl(&c->pscanned, 0ull);
+    st_ctrl(&c->prows, 0ull);
+    st_ctrl(&c->ppairs, 0ull);
+    st_ctrl(&c->rounds, 0ll);
+    st_ctrl(&c->done, 0);
+    st_ctrl(&c->status, 0);
+  }
+  bool live = false;
   int dir = 0;
-  if (P.mode == CGBFS_DIR_BOTTOMUP) dir = 1;
-  else if (P.mode == CGBFS_DIR_BEAMER && (double)mf * P.alpha > (double)P.nnz) dir = 1;
-  st_ctrl(&c->dir, dir);
-  st_ctrl(&c->done, 0);
-  st_ctrl(&c->status, 0);
+  unsigned long long deg = 0;  // out-degree of lane l's root, first lane of each root only
+  if (l < 64) {
+    st_ctrl(&L->cnt[l], 0ull);
+    st_ctrl(&L->out[l], 0ull);
+    st_ctrl(&L->in[l], 0ull);
+    if (l < A.nroots) {
+      const int64_t r = roots[l];
+      const long long mu = (long long)P.nnz - (long long)(A.in_ptr[r + 1] - A.in_ptr[r]);
+      if (P.mode == CGBFS_DIR_BOTTOMUP) dir = 1;
+      st_ctrl(&L->mu[l], mu);
+      st_ctrl(&L->prev[l], 1ll);
+      live = true;
+      bool first = true;
+      for (int k = 0; k < l; k++) first = first && roots[k] != r;
+      if (first) deg = (unsigned long long)(A.out_ptr[r + 1] - A.out_ptr[r]);
+    }
+    st_ctrl(&L->dir[l], dir);
+  }
+  // the union-count policy starts like the per-lane rules, except that 64
+  // roots may already hold enough edges to pull from round 0
+  const unsigned long long mf = wave_sum_u64(deg);
+  int g = 0;
+  if (P.mode == CGBFS_DIR_BOTTOMUP) g = 1;
+  else if (P.mode == CGBFS_DIR_BEAMER && (double)mf * P.alpha > (double)P.nnz) g = 1;
+  uint64_t pm, qm;
+  if (lane_policy(P.mode)) {
+    pm = __ballot(live && dir == 1);
+    qm = __ballot(live && dir == 0);
+  } else {
+    pm = g ? A.act : 0ull;
+    qm = g ? 0ull : A.act;
+  }
+  if (l == 0) {
+    st_ctrl(&c->gdir, g);
+    st_ctrl(&c->mf, 0ull);
+    st_ctrl(&c->live, 0ull);
+    st_ctrl(&c->segtail[0], 0ull);
+    st_ctrl(&c->segtail[1], 0ull);
+    st_ctrl(&c->prev_nf, (long long)ld_ctrl(&c->qtail[0]));
+    st_ctrl(&c->pmask, (unsigned long long)pm);
+    st_ctrl(&c->qmask, (unsigned long long)qm);
+  }
 }
 
 // Pull round over non-big vertices (thread and wavefront tiers), then the
 // big-vertex segments (one wavefront per segment, per-lane atomicMin into
 // cand[], lanes into bgot[]).
-__global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
+// pm: the lanes this round pulls (only lanes with a non-empty frontier);
+// tally_on: keep the per-lane tallies (per-lane direction policies only);
+// use_fb: the frontier bits are current (sparse frontiers only: on a dense
+// one nearly every bit is set and the extra load only adds latency).
+template <bool tally_on>
+__global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A, int d, uint64_t pm, int use_fb) {
   MsCtrl *c = A.ctrl;
   __shared__ unsigned long long red[kMsWaves];
+  __shared__ unsigned long long tl[192];
+  tally_lds_init(tl);
+  LaneTally tally;
+  const bool sym = A.symmetric != 0;
   const uint64_t *fr = A.fw[d & 1];
   uint64_t *nx = A.fw[(d + 1) & 1];
   const int16_t lev = (int16_t)(d + 1);
-  unsigned long long scanned = 0, mf = 0, rows = 0, pairs = 0;
+  // per-thread counts stay far below 2^32 (a thread owns a few rows per
+  // chunk, each shorter than kSeg); block sums are 64-bit
+  uint32_t scanned = 0, rows = 0, pairs = 0, mf = 0, nf = 0;
+  uint64_t live = 0;
   const int lane = lane_id();
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t wave = gtid / kWave, nwaves = stride / kWave;
+  const int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int32_t nwaves = (int32_t)((gridDim.x * blockDim.x) / kWave);
   // First hits of the current chunk, as offsets into the row (rows here are
   // shorter than kSeg), indexed [lane bit][row] so the per-bit flush below
   // reads one conflict-free row of LDS and stores 64 consecutive vertices.
-  __shared__ uint16_t ptile[kMsWaves][kWave][kWave];
-  uint16_t (*pt)[kWave] = ptile[threadIdx.x / kWave];
-  unsigned long long nf = 0;
+  __shared__ TileOff ptile[kMsWaves][kWave][kWave];
+  TileOff (*pt)[kWave] = ptile[threadIdx.x / kWave];
   // Chunks of 64 consecutive vertices per wavefront.  The rows of the chunk
   // that still need lanes (big rows excluded) are laid end to end in a
   // virtual edge range [0, total); each step every lane takes one virtual
@@ -469,16 +670,16 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
   // gathers the frontier word, and a segmented prefix-OR (segments = rows)
   // gives each (row, lane) its first -- lowest-id -- hit.  Row state (got)
   // lives in the lane that owns the row.
-  const int64_t nch = (A.n + kWave - 1) / kWave;
-  for (int64_t ch = wave; ch < nch; ch += nwaves) {
-    const int64_t v = ch * kWave + lane;
+  const int32_t nch = (int32_t)((A.n + kWave - 1) / kWave);
+  for (int32_t ch = wave; ch < nch; ch += nwaves) {
+    const int32_t v = ch * kWave + lane;  // vertex ids are int32
     uint64_t vw = 0, need = 0;
     int64_t rb = 0;
     int32_t len = 0, sv_me = 0;
     if (v < A.n && !A.pin.is_big[v]) {
       sv_me = A.slot_of[v];
       vw = A.vis[sv_me];
-      need = A.act & ~vw;
+      need = pm & ~vw;
       if (need) {
         rb = A.in_ptr[v];
         len = (int32_t)(A.in_ptr[v + 1] - rb);  // < kSeg for non-big rows
@@ -523,16 +724,22 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
         idx[k] = -1;
         if (ve < total && (nrk[k] & ~gr)) idx[k] = A.in_slot[rbase + (ve - rsk[k])];
       }
+      // sparse frontier: frontier bits first (a small, cached array), then
+      // the frontier words of the entries whose bit is set
+      uint32_t fbw[kUnroll];
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) fbw[k] = (use_fb && idx[k] >= 0) ? A.fb[idx[k] >> 5] : ~0u;
 #pragma unroll
       for (int k = 0; k < kUnroll; k++) {
         raw[k] = 0;
         if (idx[k] >= 0) {
-          raw[k] = fr[idx[k]] & nrk[k];
           scanned++;
+          if ((fbw[k] >> (idx[k] & 31)) & 1u) raw[k] = fr[idx[k]] & nrk[k];
         }
       }
 #pragma unroll
       for (int k = 0; k < kUnroll; k++) {
+        if (!__any(raw[k] != 0)) continue;  // no hit in this step: got unchanged
         const int32_t base = base0 + k * kWave;
         const int r = rk[k], rs = rsk[k];
         uint64_t h = raw[k] & ~shfl64(got, r);
@@ -546,7 +753,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
         }
         uint64_t prev = shfl_up64(x, 1);
         uint64_t mine = h & ~(lane > head ? prev : 0ull);
-        const uint16_t off = (uint16_t)(base + lane - rs);
+        const TileOff off = (TileOff)(base + lane - rs);
         while (mine) {
           int l = __ffsll((unsigned long long)mine) - 1;
           mine &= mine - 1;
@@ -589,48 +796,67 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull(MsArgs A, int d) {
         }
       }
     }
+    int64_t od = 0;
     if (got) {
       nx[sv_me] = got;
       A.vis[sv_me] = vw | got;
-      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
-      pairs += (unsigned long long)__popcll(got);
+      od = sym ? (int64_t)len : A.out_ptr[v + 1] - A.out_ptr[v];
+      mf += (uint32_t)od;
+      live |= got;
+      pairs += (uint32_t)__popcll(got);
       nf++;
     }
+    if (tally_on) tally_words(got, clamp_deg(od), clamp_deg(len), sym, tally);
   }
-  // Big vertices: one segment per wavefront.
-  for (int64_t s = wave; s < A.pin.nseg; s += nwaves) {
+  // Big vertices: one active segment per wavefront.
+  const int64_t nact = (int64_t)ld_ctrl(&c->segtail[0]);
+  for (int64_t i = wave; i < nact; i += nwaves) {
+    const int32_t s = A.segq[0][i];
     int32_t v = A.pin.seg_v[s];
-    uint64_t need = A.act & ~A.vis[A.slot_of[v]];
+    uint64_t need = pm & ~A.vis[A.slot_of[v]];
     if (!need) continue;
     int32_t bid = A.pin.seg_bid[s];
     int64_t beg = A.pin.seg_beg[s];
     int64_t rend = A.in_ptr[v + 1];
     int64_t end = beg + kSeg < rend ? beg + kSeg : rend;
-    uint64_t gj = ms_wave_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], scanned);
+    uint64_t gj = ms_wave_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], use_fb, scanned);
     if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
   }
+  if (tally_on) tally_flush(tally, A.lanes, tl);
+  live_flush(live, &c->live);
   unsigned long long s = ms_block_sum(nf, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->nf, s);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->pnf, s);
   s = ms_block_sum(scanned, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->pscanned, s);
+  s = ms_block_sum(rows, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->prows, s);
+  s = ms_block_sum(pairs, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->ppairs, s);
   s = ms_block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
-  s = ms_block_sum(rows, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
-  s = ms_block_sum(pairs, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->pairs, s);
 }
 
 // Big vertices after a pull round: turn per-lane candidates into outputs.
-__global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int d) {
+__global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int d, int tally_on) {
   MsCtrl *c = A.ctrl;
+  __shared__ unsigned long long red[kMsWaves];
+  __shared__ unsigned long long tl[192];
+  tally_lds_init(tl);
+  LaneTally tally;
   uint64_t *nx = A.fw[(d + 1) & 1];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < A.pin.nbig; b += stride) {
-    {
-      const int32_t v = A.pin.big_v[b];
-      const uint64_t got = A.bgot[b];
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long nf = 0, pairs = 0, mf = 0;
+  uint64_t live = 0;
+  // wavefront-uniform trip count (the tally is a wavefront operation)
+  for (int64_t b0 = gtid - lane_id(); b0 < A.pin.nbig; b0 += stride) {
+    const int64_t b = b0 + lane_id();
+    uint64_t got = 0;
+    int64_t od = 0, id = 0;
+    if (b < A.pin.nbig) {
+      got = A.bgot[b];
       if (got) {
+        const int32_t v = A.pin.big_v[b];
         uint64_t lanes = got;
         while (lanes) {
           int l = __ffsll((unsigned long long)lanes) - 1;
@@ -644,12 +870,24 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int d) {
         const int32_t sv = A.slot_of[v];
         nx[sv] = got;
         A.vis[sv] |= got;
-        atomicAdd(&c->mf, (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]));
-        atomicAdd(&c->pairs, (unsigned long long)__popcll(got));
-        atomicAdd(&c->nf, 1ull);
+        id = A.in_ptr[v + 1] - A.in_ptr[v];
+        od = A.symmetric ? id : A.out_ptr[v + 1] - A.out_ptr[v];
+        mf += (unsigned long long)od;
+        live |= got;
+        pairs += (unsigned long long)__popcll(got);
+        nf++;
       }
     }
+    if (tally_on) tally_words(got, clamp_deg(od), clamp_deg(id), A.symmetric != 0, tally);
   }
+  if (tally_on) tally_flush(tally, A.lanes, tl);
+  live_flush(live, &c->live);
+  unsigned long long s = ms_block_sum(nf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->pnf, s);
+  s = ms_block_sum(pairs, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->ppairs, s);
+  s = ms_block_sum(mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
 }
 
 // Active list from the frontier words (slot order -> vertex ids), for a push
@@ -671,6 +909,57 @@ __global__ void __launch_bounds__(kMsBlock) ms_compact(MsArgs A, int d, const in
   sq.flush();
 }
 
+// This round's active segments: those of the big vertices that still need
+// a pull lane (which = 0, in view) or that are on the push lanes' frontier
+// (which = 1, out view).  One wavefront-aggregated append per wavefront.
+__global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int d, int which,
+                                                           uint64_t mask) {
+  const SegList &L = which ? A.pout : A.pin;
+  const uint64_t *fr = A.fw[d & 1];
+  unsigned long long *tail = &A.ctrl->segtail[which];
+  int32_t *out = A.segq[which];
+  const int lane = lane_id();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; b0 < L.nbig; b0 += stride) {
+    const int64_t b = b0 + lane;
+    int32_t first = 0, cnt = 0;
+    if (b < L.nbig) {
+      const int32_t sv = A.slot_of[L.big_v[b]];
+      const bool on = which ? (fr[sv] & mask) != 0 : (mask & ~A.vis[sv]) != 0;
+      if (on) {
+        first = L.seg_first[b];
+        cnt = L.seg_first[b + 1] - first;
+      }
+    }
+    int32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      int32_t y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    const int32_t total = __shfl(incl, kWave - 1);
+    if (total == 0) continue;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(tail, (unsigned long long)total);
+    base = __shfl(base, 0) + (unsigned long long)(incl - cnt);
+    for (int32_t k = 0; k < cnt; k++) out[base + k] = first + k;
+  }
+}
+
+// Frontier bits from the frontier words: a wavefront ballots 64 consecutive
+// slots and stores the two bitmap words (whole words: no atomics, and no
+// clearing between rounds).
+__global__ void __launch_bounds__(kMsBlock) ms_mark(const uint64_t *__restrict__ fr, int64_t n,
+                                                    uint32_t *__restrict__ fb) {
+  const int lane = lane_id();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; s0 < n; s0 += stride) {
+    const int64_t s = s0 + lane;
+    const uint64_t m = __ballot(s < n && fr[s] != 0);
+    if (lane < 2) fb[(s0 >> 5) + lane] = (uint32_t)(m >> (32 * lane));
+  }
+}
+
 // Clears a whole frontier-word buffer (after a round whose frontier has no
 // list).
 __global__ void __launch_bounds__(kMsBlock) ms_clear_all(uint64_t *w, int64_t n) {
@@ -679,9 +968,11 @@ __global__ void __launch_bounds__(kMsBlock) ms_clear_all(uint64_t *w, int64_t n)
     w[k] = 0;
 }
 
-// Push pass (phase 0: OR into nx and append first writers; phase 1: offer
-// predecessors to the new pairs with per-lane atomicMin).
-__global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) {
+// Push pass for the lanes in qm (phase 0: OR into nx and append the first
+// writer of qm bits; phase 1: offer predecessors to the new pairs with
+// per-lane atomicMin).  nx may already hold bits the pull found this round;
+// they belong to other lanes.
+__global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase, uint64_t qm) {
   MsCtrl *c = A.ctrl;
   __shared__ unsigned long long red[kMsWaves];
   const uint64_t *fr = A.fw[d & 1];
@@ -699,9 +990,9 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
   __shared__ unsigned long long lq_base;
   StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kMsLdsQueue, tail, QN};
   if (phase == 0) sq.init();
-  // Work items: queue entries (skipping big vertices), then big segments of
-  // active big vertices.  Both loops are wavefront-uniform.
-  const int64_t nitems = nq + A.pout.nseg;
+  // Work items: queue entries (skipping big vertices), then the segments of
+  // the big frontier vertices.  Both loops are wavefront-uniform.
+  const int64_t nitems = nq + (int64_t)ld_ctrl(&c->segtail[1]);
   for (int64_t it = wave; it < nitems; it += nwaves) {
     int32_t u;
     int64_t beg, end;
@@ -711,13 +1002,13 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
       beg = A.out_ptr[u];
       end = A.out_ptr[u + 1];
     } else {
-      int64_t s = it - nq;
+      const int32_t s = A.segq[1][it - nq];
       u = A.pout.seg_v[s];
       beg = A.pout.seg_beg[s];
       int64_t rend = A.out_ptr[u + 1];
       end = beg + kSeg < rend ? beg + kSeg : rend;
     }
-    uint64_t f = fr[A.slot_of[u]];
+    const uint64_t f = fr[A.slot_of[u]] & qm;
     if (!f) continue;
     if (lane == 0) {
       scanned += (unsigned long long)(end - beg);
@@ -734,7 +1025,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
           uint64_t nv = f & ~A.vis[sw];
           if (nv) {
             uint64_t old = atomicOr((unsigned long long *)&nx[sw], (unsigned long long)nv);
-            first = (old == 0);
+            first = (old & qm) == 0;
           }
         } else {
           uint64_t nv = f & nx[sw];
@@ -755,36 +1046,60 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase) 
   if (phase == 0 && threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
-// Between the push passes: mark new pairs (parent slot = +inf for the
-// atomicMin of pass 2, level = d+1), fold them into vis.
-__global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int d) {
+// Between the push passes: mark the new qm pairs (parent slot = +inf for the
+// atomicMin of pass 2, level = d+1), fold them into vis.  A vertex counts
+// towards nf and mf only if the pull reached no lane of it this round.
+__global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int d, uint64_t qm, int tally_on) {
   MsCtrl *c = A.ctrl;
   __shared__ unsigned long long red[kMsWaves];
+  __shared__ unsigned long long tl[192];
+  tally_lds_init(tl);
+  LaneTally tally;
   const uint64_t *nx = A.fw[(d + 1) & 1];
   const int32_t *QN = A.q[(d + 1) & 1];
   const int64_t nq = (int64_t)ld_ctrl(&c->qtail[(d + 1) & 1]);
-  unsigned long long mf = 0, pairs = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t v = QN[i];
-    const int32_t sv = A.slot_of[v];
-    uint64_t nw = nx[sv];
-    uint64_t lanes = nw;
-    while (lanes) {
-      int l = __ffsll((unsigned long long)lanes) - 1;
-      lanes &= lanes - 1;
-      int64_t o = (int64_t)l * A.n + v;
-      A.parents[o] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
-      A.levels[o] = (int16_t)(d + 1);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long nf = 0, pairs = 0, mf = 0;
+  uint64_t live = 0;
+  // wavefront-uniform trip count (the tally is a wavefront operation)
+  for (int64_t i0 = gtid - lane_id(); i0 < nq; i0 += stride) {
+    const int64_t i = i0 + lane_id();
+    uint64_t nw = 0;
+    int64_t od = 0, id = 0;
+    if (i < nq) {
+      const int32_t v = QN[i];
+      const int32_t sv = A.slot_of[v];
+      const uint64_t all = nx[sv];
+      nw = all & qm;
+      uint64_t lanes = nw;
+      while (lanes) {
+        int l = __ffsll((unsigned long long)lanes) - 1;
+        lanes &= lanes - 1;
+        int64_t o = (int64_t)l * A.n + v;
+        A.parents[o] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
+        A.levels[o] = (int16_t)(d + 1);
+      }
+      A.vis[sv] |= nw;
+      od = A.out_ptr[v + 1] - A.out_ptr[v];
+      id = A.symmetric ? od : A.in_ptr[v + 1] - A.in_ptr[v];
+      if ((all & ~qm) == 0) {
+        nf++;
+        mf += (unsigned long long)od;
+      }
+      live |= nw;
+      pairs += (unsigned long long)__popcll(nw);
     }
-    A.vis[sv] |= nw;
-    mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
-    pairs += (unsigned long long)__popcll(nw);
+    if (tally_on) tally_words(nw, clamp_deg(od), clamp_deg(id), A.symmetric != 0, tally);
   }
-  unsigned long long s = ms_block_sum(mf, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  if (tally_on) tally_flush(tally, A.lanes, tl);
+  live_flush(live, &c->live);
+  unsigned long long s = ms_block_sum(nf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->nf, s);
   s = ms_block_sum(pairs, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->pairs, s);
+  s = ms_block_sum(mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
 }
 
 // Clears the level-d frontier words (listed in Q[d]) so that buffer is zero
@@ -799,87 +1114,155 @@ __global__ void __launch_bounds__(kMsBlock) ms_clear(MsArgs A, int d) {
     fr[A.slot_of[Q[i]]] = 0;
 }
 
-__global__ void ms_decide(MsArgs A, int d, MsParams P) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wavefront; ends round d.  Lanes that reached nothing this round drop
+// out of the next one.  CGBFS_DIR_BEAMER moves every lane together by the
+// union counts (nf = vertices with a new lane, mf = their out-degree sum);
+// CGBFS_DIR_TAU and CGBFS_DIR_BEAMER_LANES apply the single-root rules
+// (bfs.hip's bfs_decide) to each lane's own tallies, thread l for lane l.
+// Thread 0 records the round and ends the search once nothing was reached.
+// relist: the round's new vertices are not all listed (a pull part lists
+// none), so the next round's push part rebuilds the list from an empty tail.
+__global__ void ms_decide(MsArgs A, int d, MsParams P, int relist) {
+  if (blockIdx.x != 0) return;
   MsCtrl *c = A.ctrl;
-  const int cur = ld_ctrl(&c->dir);
-  // push rounds list the new active vertices; pull rounds only count them
-  const long long nf =
-      (long long)(cur == 0 ? ld_ctrl(&c->qtail[(d + 1) & 1]) : ld_ctrl(&c->nf));
-  const long long mf = (long long)ld_ctrl(&c->mf);
+  MsLane *L = A.lanes;
+  const int l = threadIdx.x;
+  const uint64_t live = (uint64_t)ld_ctrl(&c->live) & A.act;
+  const long long pnf = (long long)ld_ctrl(&c->pnf);
+  const long long nf = pnf + (long long)ld_ctrl(&c->nf);
+  int g = ld_ctrl(&c->gdir);
+  uint64_t pm = 0, qm = 0;
+  if (lane_policy(P.mode)) {
+    bool lv = false;
+    int dir = 0;
+    if (l < A.nroots) {
+      const long long cnt = (long long)ld_ctrl(&L->cnt[l]);
+      const long long mfl = (long long)ld_ctrl(&L->out[l]);
+      const long long mu = ld_ctrl(&L->mu[l]) - (long long)ld_ctrl(&L->in[l]);
+      const long long prev = ld_ctrl(&L->prev[l]);
+      dir = ld_ctrl(&L->dir[l]);
+      st_ctrl(&L->cnt[l], 0ull);
+      st_ctrl(&L->out[l], 0ull);
+      st_ctrl(&L->in[l], 0ull);
+      if (P.mode == CGBFS_DIR_TAU) {
+        if (dir == 0 && cnt > P.tau) dir = 1;  // one-way, bool_pe.csl:1078-1088
+      } else if (dir == 0) {
+        if ((double)mfl > (double)mu / P.alpha) dir = 1;
+      } else if ((double)cnt < (double)A.n / P.beta && cnt < prev) {
+        dir = 0;
+      }
+      st_ctrl(&L->mu[l], mu);
+      st_ctrl(&L->prev[l], cnt);
+      st_ctrl(&L->dir[l], dir);
+      lv = (live >> l) & 1ull;
+    }
+    pm = __ballot(lv && dir == 1);
+    qm = __ballot(lv && dir == 0);
+  } else {
+    if (P.mode == CGBFS_DIR_TOPDOWN) {
+      g = 0;
+    } else if (P.mode == CGBFS_DIR_BOTTOMUP) {
+      g = 1;
+    } else if (g == 0) {
+      if ((double)ld_ctrl(&c->mf) * P.alpha > (double)P.nnz) g = 1;
+    } else if ((double)nf * P.beta < (double)A.n && nf < ld_ctrl(&c->prev_nf)) {
+      g = 0;
+    }
+    pm = g ? live : 0ull;
+    qm = g ? 0ull : live;
+  }
+  if (l != 0) return;
+  const unsigned long long cur_pm = ld_ctrl(&c->pmask), cur_qm = ld_ctrl(&c->qmask);
   if (d < kMsHist) {
-    A.hist->nf[d] = nf;
-    A.hist->scanned[d] = (long long)ld_ctrl(&c->scanned);
-    A.hist->rows[d] = (long long)ld_ctrl(&c->rows);
-    A.hist->pairs[d] = (long long)ld_ctrl(&c->pairs);
-    A.hist->dir[d] = cur;
+    MsHist *H = A.hist;
+    H->nf[d] = nf;
+    H->pnf[d] = pnf;
+    H->pscanned[d] = (long long)ld_ctrl(&c->pscanned);
+    H->prows[d] = (long long)ld_ctrl(&c->prows);
+    H->ppairs[d] = (long long)ld_ctrl(&c->ppairs);
+    H->scanned[d] = H->pscanned[d] + (long long)ld_ctrl(&c->scanned);
+    H->rows[d] = H->prows[d] + (long long)ld_ctrl(&c->rows);
+    H->pairs[d] = H->ppairs[d] + (long long)ld_ctrl(&c->pairs);
+    H->plane[d] = __popcll(cur_pm);
+    H->dir[d] = cur_pm ? (cur_qm ? 2 : 1) : 0;
   }
   st_ctrl(&c->nf, 0ull);
-  st_ctrl(&c->mf, 0ull);
   st_ctrl(&c->scanned, 0ull);
   st_ctrl(&c->rows, 0ull);
   st_ctrl(&c->pairs, 0ull);
+  st_ctrl(&c->pnf, 0ull);
+  st_ctrl(&c->pscanned, 0ull);
+  st_ctrl(&c->prows, 0ull);
+  st_ctrl(&c->ppairs, 0ull);
+  st_ctrl(&c->mf, 0ull);
+  st_ctrl(&c->live, 0ull);
+  st_ctrl(&c->segtail[0], 0ull);
+  st_ctrl(&c->segtail[1], 0ull);
   st_ctrl(&c->qtail[d & 1], 0ull);
+  if (relist) st_ctrl(&c->qtail[(d + 1) & 1], 0ull);
   st_ctrl(&c->rounds, (long long)(d + 1));
+  st_ctrl(&c->prev_nf, nf);
+  st_ctrl(&c->gdir, g);
+  st_ctrl(&c->pmask, (unsigned long long)pm);
+  st_ctrl(&c->qmask, (unsigned long long)qm);
   if (nf == 0) {
     st_ctrl(&c->done, 1);
     return;
   }
-  int dir = cur;
-  switch (P.mode) {
-    case CGBFS_DIR_TOPDOWN: dir = 0; break;
-    case CGBFS_DIR_BOTTOMUP: dir = 1; break;
-    case CGBFS_DIR_TAU:
-      if (dir == 0 && nf > P.tau) dir = 1;
-      break;
-    default:
-      if (dir == 0) {
-        if ((double)mf * P.alpha > (double)P.nnz) dir = 1;
-      } else {
-        if ((double)nf * P.beta < (double)A.n && nf < ld_ctrl(&c->prev_nf)) dir = 0;
-      }
-  }
-  st_ctrl(&c->prev_nf, nf);
-  st_ctrl(&c->dir, dir);
   if (d + 1 >= P.max_levels) {
     st_ctrl(&c->status, (int)CGBFS_ETIMEOUT);
     st_ctrl(&c->done, 1);
   }
 }
 
-// Final pass: -1 into every (lane, vertex) pair no lane word reached.
+// Final pass: -1 into every (lane, vertex) pair no lane word reached.  A
+// wavefront owns 64 consecutive vertices and walks the lane bits missing in
+// any of them, so each store instruction covers 64 consecutive entries of
+// one lane's output row.
 __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < A.n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t miss = A.act & ~A.vis[A.slot_of[v]];
-    while (miss) {
-      int l = __ffsll((unsigned long long)miss) - 1;
-      miss &= miss - 1;
-      int64_t o = (int64_t)l * A.n + v;
-      A.parents[o] = -1;
-      A.levels[o] = -1;
+  const int lane = lane_id();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < A.n; v0 += stride) {
+    const int64_t v = v0 + lane;
+    const uint64_t miss = v < A.n ? A.act & ~A.vis[A.slot_of[v]] : 0ull;
+    uint64_t bits = wave_or64(miss);
+    while (bits) {
+      const int l = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      if ((miss >> l) & 1ull) {
+        const int64_t o = (int64_t)l * A.n + v;
+        A.parents[o] = -1;
+        A.levels[o] = -1;
+      }
     }
   }
 }
 
-// GTEPS m per lane: sum of the per-vertex weight over each lane's reached set.
+// GTEPS m per lane: sum of the per-vertex weight over each lane's reached
+// set.  Per wavefront of 64 vertices, one 64-bit wavefront sum per lane bit
+// present; lane l of the wavefront keeps lane bit l's total.
 __global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const int32_t *slot_of,
                                                      const int32_t *w, int64_t n, int nroots,
                                                      unsigned long long *out) {
   __shared__ unsigned long long acc[64];
   if (threadIdx.x < 64) acc[threadIdx.x] = 0;
   __syncthreads();
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    unsigned long long wv = (unsigned long long)w[v];
-    if (!wv) continue;
-    uint64_t b = vis[slot_of[v]];
-    while (b) {
-      int l = __ffsll((unsigned long long)b) - 1;
-      b &= b - 1;
-      atomicAdd(&acc[l], wv);
+  const int lane = lane_id();
+  unsigned long long mine = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < n; v0 += stride) {
+    const int64_t v = v0 + lane;
+    const unsigned long long wv = v < n ? (unsigned long long)w[v] : 0ull;
+    const uint64_t b = wv ? vis[slot_of[v]] : 0ull;
+    uint64_t bits = wave_or64(b);
+    while (bits) {
+      const int l = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      const unsigned long long s = wave_sum_u64(((b >> l) & 1ull) ? wv : 0ull);
+      if (lane == l) mine += s;
     }
   }
+  if (mine) atomicAdd(&acc[lane], mine);
   __syncthreads();
   if (threadIdx.x < (unsigned)nroots && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
 }
@@ -903,7 +1286,8 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
     if (roots[l] < 0 || roots[l] >= g->n) return set_error(CGBFS_EINVAL, "root out of range");
   cgbfs_options o;
   fill_options(opt, g->n, &o);
-  if (o.direction < 0 || o.direction > 3) return set_error(CGBFS_EINVAL, "unknown direction mode");
+  if (o.direction < 0 || o.direction > CGBFS_DIR_BEAMER_LANES)
+    return set_error(CGBFS_EINVAL, "unknown direction mode");
   if (o.max_levels > 32767) o.max_levels = 32767;  // int16 levels
   CG_HIP(hipSetDevice(g->device));
   MsWorkspace ws;
@@ -932,10 +1316,12 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   A.vis = ws.vis;
   A.fw[0] = ws.fw[0];
   A.fw[1] = ws.fw[1];
+  A.fb = ws.fb;
   A.q[0] = ws.q[0];
   A.q[1] = ws.q[1];
   A.ctrl = ws.ctrl;
   A.hist = ws.hist;
+  A.lanes = ws.lanes;
   A.pin = md->pin;
   A.pout = md->pout;
   A.slot_of = md->slot_of;
@@ -943,9 +1329,12 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   A.out_slot = md->out_slot;
   A.cand = ws.cand;
   A.bgot = ws.bgot;
+  A.segq[0] = ws.segq[0];
+  A.segq[1] = ws.segq[1];
   A.n = g->n;
   A.nroots = nroots;
   A.act = nroots == 64 ? ~0ull : ((1ull << nroots) - 1ull);
+  A.symmetric = g->in_ptr == g->out_ptr ? 1 : 0;
 
   const int blocks = grid_blocks(g->device, o.grid_mult);
   std::vector<hipEvent_t> evs;
@@ -979,36 +1368,52 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   MsCtrl hc;
   e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  // One host poll per round: only the chosen direction's kernels launch,
-  // and each round's traversal kernels are bracketed by a pair of events.
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> rev;
-  std::vector<int> rdir;
+  // One host poll per round: the round pulls the lanes in pmask, then pushes
+  // the lanes in qmask; each part is bracketed by events.
+  struct RoundEvents { hipEvent_t a, p, b; };
+  std::vector<RoundEvents> rev;
   int64_t d = 0;
-  // listed: the current frontier's vertices are listed in Q[d&1] (the roots,
-  // or the output of a push round).  A pull round leaves no list; a push
-  // round after it first compacts one from the frontier words.
+  // listed: the current frontier's vertices are all listed in Q[d&1] (the
+  // roots, or the output of a round without a pull part).  The pull lists
+  // nothing; a push part after it first compacts a list from the frontier
+  // words.
   bool listed = true;
+  const int tally = lane_policy(P.mode) ? 1 : 0;
+  auto sblocks = [&](int64_t nb) {
+    int64_t k = (nb + kMsBlock - 1) / kMsBlock;
+    return (unsigned)(k < blocks ? k : blocks);
+  };
   while (e == hipSuccess && !hc.done) {
-    hipEvent_t ra, rb;
-    if ((e = new_event(&ra)) != hipSuccess || (e = new_event(&rb)) != hipSuccess) break;
-    hipEventRecord(ra, st);
-    if (hc.dir == 1) {
-      ms_pull<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
-      if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, (int)d);
-    } else {
-      if (!listed) ms_compact<<<blocks, kMsBlock, 0, st>>>(A, (int)d, md->vertex_of);
-      ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 0);
-      ms_prep<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
-      ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 1);
-      listed = true;
+    RoundEvents re;
+    if ((e = new_event(&re.a)) != hipSuccess || (e = new_event(&re.p)) != hipSuccess ||
+        (e = new_event(&re.b)) != hipSuccess)
+      break;
+    const uint64_t pm = hc.pmask, qm = hc.qmask;
+    hipEventRecord(re.a, st);
+    if (pm) {
+      // frontier bits pay off when at most 1/8 of the vertices are active
+      const int use_fb = hc.prev_nf * 8 <= g->n ? 1 : 0;
+      if (use_fb) ms_mark<<<blocks, kMsBlock, 0, st>>>(A.fw[d & 1], g->n, A.fb);
+      if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, (int)d, 0, pm);
+      if (tally) ms_pull<true><<<blocks, kMsBlock, 0, st>>>(A, (int)d, pm, use_fb);
+      else ms_pull<false><<<blocks, kMsBlock, 0, st>>>(A, (int)d, pm, use_fb);
+      if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, (int)d, tally);
     }
-    hipEventRecord(rb, st);
-    rev.push_back({ra, rb});
-    rdir.push_back(hc.dir);
+    hipEventRecord(re.p, st);
+    if (qm) {
+      if (!listed) ms_compact<<<blocks, kMsBlock, 0, st>>>(A, (int)d, md->vertex_of);
+      listed = true;
+      if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, (int)d, 1, qm);
+      ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 0, qm);
+      ms_prep<<<blocks, kMsBlock, 0, st>>>(A, (int)d, qm, tally);
+      ms_push<<<blocks, kMsBlock, 0, st>>>(A, (int)d, 1, qm);
+    }
+    hipEventRecord(re.b, st);
+    rev.push_back(re);
     if (listed) ms_clear<<<blocks, kMsBlock, 0, st>>>(A, (int)d);
     else ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A.fw[d & 1], g->n);
-    listed = (hc.dir == 0);  // the next frontier is listed iff this round pushed
-    ms_decide<<<1, 64, 0, st>>>(A, (int)d, P);
+    listed = (pm == 0);  // the next frontier is listed iff nothing pulled
+    ms_decide<<<1, 64, 0, st>>>(A, (int)d, P, listed ? 0 : 1);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -1070,10 +1475,17 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
         if (stats->edges_scanned) stats->edges_scanned[r] = H->scanned[r];
         if (stats->rows_scanned) stats->rows_scanned[r] = H->rows[r];
         if (stats->pairs_found) stats->pairs_found[r] = H->pairs[r];
-        if (stats->level_ms && r < (int64_t)rev.size()) {
-          float rm = 0;
-          hipEventElapsedTime(&rm, rev[r].first, rev[r].second);
-          stats->level_ms[r] = rm;
+        if (stats->pull_lanes) stats->pull_lanes[r] = H->plane[r];
+        if (stats->pull_nf) stats->pull_nf[r] = H->pnf[r];
+        if (stats->pull_rows) stats->pull_rows[r] = H->prows[r];
+        if (stats->pull_edges) stats->pull_edges[r] = H->pscanned[r];
+        if (stats->pull_pairs) stats->pull_pairs[r] = H->ppairs[r];
+        if (r < (int64_t)rev.size()) {
+          float rm = 0, pms = 0;
+          hipEventElapsedTime(&rm, rev[r].a, rev[r].b);
+          hipEventElapsedTime(&pms, rev[r].a, rev[r].p);
+          if (stats->level_ms) stats->level_ms[r] = rm;
+          if (stats->pull_ms) stats->pull_ms[r] = pms;
         }
       }
     }

@@ -38,6 +38,13 @@ class SearchStats:
   level_ms: np.ndarray
   visited_count: int = -1
   m_edges: int = -1
+  # multi-source: the pull part of each round (the push part is the rest)
+  pull_lanes: np.ndarray = None
+  pull_nf: np.ndarray = None
+  pull_rows: np.ndarray = None
+  pull_edges: np.ndarray = None
+  pull_pairs: np.ndarray = None
+  pull_ms: np.ndarray = None
 
 
 @dataclass
@@ -88,6 +95,12 @@ class _StatsBuf:
     s.rows_scanned = self.rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     s.pairs_found = self.pairs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
     s.level_ms = self.level_ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    self.pull = {k: np.zeros(cap, np.int64) for k in ("pull_lanes", "pull_nf", "pull_rows",
+                                                     "pull_edges", "pull_pairs")}
+    self.pull["pull_ms"] = np.zeros(cap, np.float64)
+    for k, a in self.pull.items():
+      ct = ctypes.c_double if a.dtype == np.float64 else ctypes.c_int64
+      setattr(s, k, a.ctypes.data_as(ctypes.POINTER(ct)))
     self.s = s
 
   def result(self):
@@ -96,7 +109,8 @@ class _StatsBuf:
                        float(self.s.output_ms), self.nf[:r].copy(), self.dir[:r].copy(),
                        self.scanned[:r].copy(), self.rows[:r].copy(), self.pairs[:r].copy(),
                        self.level_ms[:r].copy(),
-                       int(self.s.visited_count), int(self.s.m_edges))
+                       int(self.s.visited_count), int(self.s.m_edges),
+                       **{k: a[:r].copy() for k, a in self.pull.items()})
 
 
 class DeviceGraph:

@@ -46,6 +46,12 @@ extern "C" {
 #define CGBFS_DIR_BOTTOMUP 2 /* pull every level                                 */
 #define CGBFS_DIR_TAU 3      /* reference rule: one-way push->pull once nf > tau
                                 (bool_pe.csl:1078-1088, run_bfs.py:239-243)      */
+#define CGBFS_DIR_BEAMER_LANES 4 /* multi-source: alpha/beta applied to each lane's
+                                own counts, a round pulls some lanes and pushes
+                                the rest; a single root: CGBFS_DIR_BEAMER        */
+/* In a multi-source search CGBFS_DIR_BEAMER decides from the union counts
+ * (vertices with a new lane, their out-degree sum) and moves all lanes
+ * together; CGBFS_DIR_TAU applies tau to each lane's own count. */
 
 typedef struct cgbfs_graph cgbfs_graph;
 
@@ -80,6 +86,16 @@ typedef struct {
   double output_ms;         /* device->host output copy (host outputs)     */
   int64_t visited_count;    /* single-root only                            */
   int64_t m_edges;          /* GTEPS m for a single root (bfs_timing.py:390-395) */
+  /* Multi-source only: a round pulls some lanes and pushes the others
+   * (direction_history[r] = 0 push only, 1 pull only, 2 both).  The arrays
+   * below give the pull part of round r, and the push part is the total
+   * minus it.  Any of them may be NULL. */
+  int64_t *pull_lanes;      /* lanes that pulled in round r                */
+  int64_t *pull_nf;         /* vertices the pull reached in round r        */
+  int64_t *pull_rows;       /* adjacency lists the pull opened             */
+  int64_t *pull_edges;      /* adjacency entries the pull read             */
+  int64_t *pull_pairs;      /* (lane, vertex) pairs the pull reached       */
+  double *pull_ms;          /* round r's pull kernels (HIP events)         */
 } cgbfs_stats;
 
 /* --- library ------------------------------------------------------------ */
@@ -137,7 +153,9 @@ CGBFS_API int cgbfs_bfs(cgbfs_graph *g, int64_t source, int32_t *parents, int32_
  * is exactly cgbfs_bfs(roots[l]) (levels cast to int16; a search deeper
  * than 32767 levels fails with CGBFS_EINVAL).  m_lanes (may be NULL):
  * int64[nroots], GTEPS m per lane.  stats->nf_history counts vertices with
- * at least one newly reached lane per round. */
+ * at least one newly reached lane per round.  A round pulls the lanes in
+ * pull mode and pushes the rest (see the direction policies); lanes whose
+ * frontier is empty take no part. */
 CGBFS_API int cgbfs_msbfs(cgbfs_graph *g, const int64_t *roots, int32_t nroots, int32_t *parents,
                 int16_t *levels, int32_t on_device, const cgbfs_options *opt,
                 cgbfs_stats *stats, int64_t *m_lanes);

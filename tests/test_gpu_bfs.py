@@ -142,14 +142,40 @@ def test_msbfs_lanes_equal_oracle(scale, ef, seed, directed, nroots):
   roots[0] = int(np.argmax(deg))
   if nroots > 2:
     roots[1] = roots[2]  # a repeated root
+  oracle_levels = np.stack([O.bfs(g, int(s))[0] for s in roots])
   with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
-    for d in DIRECTIONS:
+    for d in DIRECTIONS + ["beamer_lanes"]:
       r = dg.msbfs(roots, direction=d)
       for l, s in enumerate(roots):
         lev, par, _, rounds = O.bfs(g, int(s))
         np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16), err_msg=f"lane {l} dir {d}")
         np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d}")
         assert r.m_lanes[l] == O.m_edges(g, lev)
+      _check_ms_stats(r.stats, oracle_levels, d)
+
+
+def _check_ms_stats(st, levs, direction):
+  """Round accounting of a multi-source pass against the lanes' oracle
+  levels: round r reaches the (lane, vertex) pairs at level r+1, nf counts
+  vertices with at least one such lane, the last round finds nothing; the
+  pull part of each round is a part of it; the union-count and fixed
+  policies never mix directions within a round."""
+  maxl = int(levs.max())
+  assert st.rounds_completed == maxl + 1
+  for r in range(maxl + 1):
+    hit = levs == r + 1
+    assert st.nf_history[r] == int(hit.any(axis=0).sum()), f"nf round {r}"
+    assert st.pairs_found[r] == int(hit.sum()), f"pairs round {r}"
+  assert np.all(st.pull_nf <= st.nf_history) and np.all(st.pull_pairs <= st.pairs_found)
+  assert np.all(st.pull_edges <= st.edges_scanned) and np.all(st.pull_rows <= st.rows_scanned)
+  assert set(st.direction_history.tolist()) <= {0, 1, 2}
+  assert np.all((st.pull_lanes > 0) == (st.direction_history > 0))
+  if direction in ("beamer", "topdown", "bottomup"):
+    assert 2 not in st.direction_history  # these move every lane together
+  if direction == "topdown":
+    assert not st.pull_lanes.any()
+  if direction == "bottomup":
+    assert np.all(st.direction_history == 1)
 
 
 def test_c2_scale21_sampled_roots():
