@@ -117,8 +117,9 @@ def main():
   batched_aggregate = m.sum() / t_step / 1e9
   lane_hmean = harmonic_mean(m / t_step / 1e9)
 
-  # Roofline of the dominant kernel: the pull part of the multi-source
-  # rounds (ms_pull + ms_pull_fin), timed by the library's HIP events.
+  # Roofline of the dominant kernel: the multi-source pull (ms_pull and its
+  # big-vertex epilogue ms_pull_fin), timed by the library's HIP events on
+  # the graph's stream.
   st = last.stats
   for i in range(int(st.rounds_completed)):
     log(f"  round {i}: dir {st.direction_history[i]} pull lanes {st.pull_lanes[i]:2d} "
@@ -138,7 +139,7 @@ def main():
       traffic = json.load(f).get("hbm_bytes_per_launch")
   roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
               "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-              "kernel": "ms_pull", "launches": npull,
+              "kernel": "ms_pull+ms_pull_fin", "launches": npull,
               "avg_launch_ms": round(tpull * 1e3 / max(npull, 1), 4),
               "algorithmic_bytes_per_launch": int(byts / max(npull, 1))}
   log(f"pull rounds: {npull}, {tpull*1e3:.3f} ms, {byts/1e9:.2f} GB algorithmic -> "

@@ -53,6 +53,7 @@ struct DeviceGraph {
   int32_t *m_weight = nullptr;
   int64_t total_out_edges = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // second stream, for work that overlaps a search
   // Lazily sized single-root workspace.
   void *ws1 = nullptr;
   size_t ws1_bytes = 0;

@@ -276,6 +276,7 @@ static void graph_free(DeviceGraph *g) {
   if (g->msdata && g->msdata_free) g->msdata_free(g->msdata);
   if (g->host_flag) hipHostFree(g->host_flag);
   if (g->stream) hipStreamDestroy(g->stream);
+  if (g->side) hipStreamDestroy(g->side);
   // DeviceGraph is the first and only member of the handle.
   delete reinterpret_cast<cgbfs_graph *>(g);
 }

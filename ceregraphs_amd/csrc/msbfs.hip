@@ -55,7 +55,7 @@ constexpr int kMsHist = 4096;
 constexpr int64_t kSeg = 256;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
-constexpr int kFlush = 4;           // lane bits per batch of the pull's output flush
+constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
 // First-hit offsets within a wavefront-tier row (< kSeg) fit in a byte.
 using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
@@ -119,6 +119,7 @@ struct SegList {
   int32_t *seg_v = nullptr;    // vertex of each segment
   int32_t *seg_bid = nullptr;  // big-vertex index of each segment
   int64_t *seg_beg = nullptr;  // first adjacency entry of each segment
+  int32_t *seg_len = nullptr;  // its length (<= kSeg)
   int32_t *big_v = nullptr;    // big vertices, by index
   int32_t *seg_first = nullptr; // [nbig + 1] first segment of each big vertex
   uint8_t *is_big = nullptr;   // per vertex
@@ -145,6 +146,8 @@ struct MsArgs {
   // round's active ones (indices into the static lists)
   SegList pin, pout;
   int32_t *segq[2];
+  uint64_t *segneed;  // [pin.nseg] lanes needed by each active pull segment
+  const uint64_t *noin;  // vertices without in-edges, one bit each
   // Vertex-state slot of each vertex (hub-first order) and the adjacency
   // arrays re-expressed in slots, entry for entry; the lane words vis/fw
   // are indexed by slot, everything else by vertex id.
@@ -164,6 +167,9 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 struct MsGraphData {
   SegList pin, pout;
   void *mem = nullptr;
+  // One bit per vertex: no in-edges, so no root but itself ever reaches it
+  // ([ceil(n/64)] words).
+  uint64_t *noin = nullptr;
   int32_t *slot_of = nullptr;   // [n]
   int32_t *vertex_of = nullptr; // [n], inverse of slot_of
   int32_t *in_slot = nullptr;   // [nnz], in view
@@ -182,6 +188,7 @@ struct MsWorkspace {
   uint32_t *cand;
   uint64_t *bgot;
   int32_t *segq[2];
+  uint64_t *segneed;
 };
 
 // ---------------------------------------------------------------------------
@@ -221,10 +228,18 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
     CG_HIP(hipMemcpy(ptr_out.data(), g->out_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
     seg_plan(ptr_out, n, sv[1], sb[1], sg[1], bv[1], bf[1], isb[1]);
   }
+  std::vector<int32_t> sl[2];
+  for (int k = 0; k < (shared ? 1 : 2); k++) {
+    const std::vector<int64_t> &ptr = k == 0 ? ptr_in : ptr_out;
+    sl[k].resize(sv[k].size());
+    for (size_t i = 0; i < sv[k].size(); i++)
+      sl[k][i] = (int32_t)std::min<int64_t>(kSeg, ptr[sv[k][i] + 1] - sg[k][i]);
+  }
   const int views = shared ? 1 : 2;
   size_t total = 0;
   for (int k = 0; k < views; k++)
     total += al(sv[k].size() * 4 + 4) + al(sb[k].size() * 4 + 4) + al(sg[k].size() * 8 + 8) +
+             al(sl[k].size() * 4 + 4) +
              al(bv[k].size() * 4 + 4) + al(bf[k].size() * 4) + al((size_t)n);
   CG_HIP(hipMalloc(&md->mem, total));
   char *p = (char *)md->mem;
@@ -235,6 +250,7 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
     L.seg_v = (int32_t *)p; p += al(sv[k].size() * 4 + 4);
     L.seg_bid = (int32_t *)p; p += al(sb[k].size() * 4 + 4);
     L.seg_beg = (int64_t *)p; p += al(sg[k].size() * 8 + 8);
+    L.seg_len = (int32_t *)p; p += al(sl[k].size() * 4 + 4);
     L.big_v = (int32_t *)p; p += al(bv[k].size() * 4 + 4);
     L.seg_first = (int32_t *)p; p += al(bf[k].size() * 4);
     L.is_big = (uint8_t *)p; p += al((size_t)n);
@@ -242,12 +258,20 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
       CG_HIP(hipMemcpy(L.seg_v, sv[k].data(), sv[k].size() * 4, hipMemcpyHostToDevice));
       CG_HIP(hipMemcpy(L.seg_bid, sb[k].data(), sb[k].size() * 4, hipMemcpyHostToDevice));
       CG_HIP(hipMemcpy(L.seg_beg, sg[k].data(), sg[k].size() * 8, hipMemcpyHostToDevice));
+      CG_HIP(hipMemcpy(L.seg_len, sl[k].data(), sl[k].size() * 4, hipMemcpyHostToDevice));
       CG_HIP(hipMemcpy(L.big_v, bv[k].data(), bv[k].size() * 4, hipMemcpyHostToDevice));
     }
     CG_HIP(hipMemcpy(L.seg_first, bf[k].data(), bf[k].size() * 4, hipMemcpyHostToDevice));
     CG_HIP(hipMemcpy(L.is_big, isb[k].data(), (size_t)n, hipMemcpyHostToDevice));
   }
   if (shared) md->pout = md->pin;
+  {
+    std::vector<uint64_t> bits((size_t)((n + 63) / 64), 0ull);
+    for (int64_t v = 0; v < n; v++)
+      if (ptr_in[v + 1] == ptr_in[v]) bits[(size_t)(v >> 6)] |= 1ull << (v & 63);
+    CG_HIP(hipMalloc(&md->noin, sizeof(uint64_t) * (bits.size() + 1)));
+    CG_HIP(hipMemcpy(md->noin, bits.data(), sizeof(uint64_t) * bits.size(), hipMemcpyHostToDevice));
+  }
   return CGBFS_OK;
 }
 
@@ -322,6 +346,7 @@ static int build_slots(DeviceGraph *g, MsGraphData *md) {
 static void free_msdata(void *q) {
   MsGraphData *m = (MsGraphData *)q;
   if (m->mem) hipFree(m->mem);
+  if (m->noin) hipFree(m->noin);
   if (m->out_slot && m->out_slot != m->in_slot) hipFree(m->out_slot);
   if (m->in_slot) hipFree(m->in_slot);
   if (m->slot_of) hipFree(m->slot_of);
@@ -350,7 +375,8 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
   size_t total = 3 * wb + fbb + 2 * qb + al(sizeof(MsCtrl)) + al(sizeof(MsHist)) + al(sizeof(MsLane)) +
                  al(sizeof(unsigned long long) * 64) + al(sizeof(int64_t) * 64) +
                  al(sizeof(uint32_t) * 64 * (nbig + 1)) + al(sizeof(uint64_t) * (nbig + 1)) +
-                 al(sizeof(int32_t) * (md->pin.nseg + 1)) + al(sizeof(int32_t) * (md->pout.nseg + 1));
+                 al(sizeof(int32_t) * (md->pin.nseg + 1)) + al(sizeof(int32_t) * (md->pout.nseg + 1)) +
+                 al(sizeof(uint64_t) * (md->pin.nseg + 1));
   if (g->wsm_bytes < total) {
     if (g->wsm) hipFree(g->wsm);
     g->wsm = nullptr;
@@ -373,7 +399,8 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
   w->cand = (uint32_t *)p; p += al(sizeof(uint32_t) * 64 * (nbig + 1));
   w->bgot = (uint64_t *)p; p += al(sizeof(uint64_t) * (nbig + 1));
   w->segq[0] = (int32_t *)p; p += al(sizeof(int32_t) * (md->pin.nseg + 1));
-  w->segq[1] = (int32_t *)p;
+  w->segq[1] = (int32_t *)p; p += al(sizeof(int32_t) * (md->pout.nseg + 1));
+  w->segneed = (uint64_t *)p;
   return CGBFS_OK;
 }
 
@@ -412,6 +439,14 @@ __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
     for (int i = 0; i < (int)(blockDim.x / kWave); i++) s += lds[i];
   __syncthreads();
   return s;
+}
+
+// Orders a wavefront's LDS writes before its later LDS reads (no block
+// barrier: the callers loop with wavefront-divergent trip counts).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
@@ -497,35 +532,51 @@ __device__ __forceinline__ void tally_lds_init(unsigned long long *lds) {
 }
 
 
-// Wavefront scan of in-neighbour entries [beg, end) for lanes `need`
-// (wave-uniform arguments): gathers lane words through in_slot and offers
-// each lane's first hit in the range (its original id) to cand[] with
-// atomicMin.  Returns the lanes found (wave-uniform).
-__device__ __forceinline__ uint64_t ms_wave_pull(const MsArgs &A, const uint64_t *fr, int64_t beg,
-                                                 int64_t end, uint64_t need, uint32_t *cand,
-                                                 bool use_fb, uint32_t &scanned) {
+constexpr int kSegSteps = (int)(kSeg / kWave);
+static_assert(kSeg % kWave == 0, "a segment is a whole number of wavefront steps");
+
+// One segment [beg, end) of an in-neighbour list, end - beg <= kSeg, for the
+// lanes `need` (wavefront-uniform arguments).  Every index, id and frontier
+// word of the segment is loaded before the scan starts, without branches
+// (lanes past the end or without a frontier bit read a cached dummy), so the
+// loads of all steps are in flight together.  Each lane's first hit in the
+// segment (its original id) is offered to cand[] with atomicMin.  Returns
+// the lanes found (wavefront-uniform).
+__device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t *fr, int64_t beg,
+                                                int64_t end, uint64_t need, uint32_t *cand,
+                                                bool use_fb, uint32_t &scanned) {
   const int lane = lane_id();
+  int32_t sl[kSegSteps], uid[kSegSteps];
+  bool ok[kSegSteps];
+  uint64_t w[kSegSteps];
+#pragma unroll
+  for (int k = 0; k < kSegSteps; k++) {
+    const int64_t e = beg + k * kWave + lane;
+    ok[k] = e < end;
+    const int64_t ee = ok[k] ? e : beg;
+    sl[k] = A.in_slot[ee];
+    uid[k] = A.in_idx[ee];
+  }
+  if (use_fb) {
+#pragma unroll
+    for (int k = 0; k < kSegSteps; k++) ok[k] = ok[k] && ((A.fb[sl[k] >> 5] >> (sl[k] & 31)) & 1u);
+  }
+#pragma unroll
+  for (int k = 0; k < kSegSteps; k++) w[k] = fr[ok[k] ? sl[k] : 0];
+  if (lane == 0) scanned += (uint32_t)(end - beg);
   uint64_t got = 0;
-  for (int64_t e0 = beg; e0 < end; e0 += kWave) {
-    int64_t e = e0 + lane;
-    uint64_t h = 0;
-    if (e < end) {
-      const int32_t s = A.in_slot[e];
-      if (!use_fb || ((A.fb[s >> 5] >> (s & 31)) & 1u)) h = fr[s] & need & ~got;
-    }
-    if (lane == 0) scanned += (uint32_t)((end - e0) < kWave ? (end - e0) : kWave);
+#pragma unroll
+  for (int k = 0; k < kSegSteps; k++) {
+    const uint64_t h = ok[k] ? w[k] & need & ~got : 0ull;
     if (!__any(h != 0)) continue;  // no frontier in-neighbour in this step
-    uint64_t inc = wave_scan_or(h);
+    const uint64_t inc = wave_scan_or(h);
     uint64_t before = shfl_up64(inc, 1);
     if (lane == 0) before = 0;
     uint64_t mine = h & ~before;
-    if (mine) {
-      const uint32_t u = (uint32_t)A.in_idx[e];
-      while (mine) {
-        int l = __ffsll((unsigned long long)mine) - 1;
-        mine &= mine - 1;
-        atomicMin(&cand[l], u);
-      }
+    while (mine) {
+      int l = __ffsll((unsigned long long)mine) - 1;
+      mine &= mine - 1;
+      atomicMin(&cand[l], (uint32_t)uid[k]);
     }
     got |= shfl64(inc, kWave - 1);
     if (got == need) break;
@@ -673,18 +724,17 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A, int d, uint64_t
   const int32_t nch = (int32_t)((A.n + kWave - 1) / kWave);
   for (int32_t ch = wave; ch < nch; ch += nwaves) {
     const int32_t v = ch * kWave + lane;  // vertex ids are int32
-    uint64_t vw = 0, need = 0;
-    int64_t rb = 0;
-    int32_t len = 0, sv_me = 0;
-    if (v < A.n && !A.pin.is_big[v]) {
-      sv_me = A.slot_of[v];
-      vw = A.vis[sv_me];
-      need = pm & ~vw;
-      if (need) {
-        rb = A.in_ptr[v];
-        len = (int32_t)(A.in_ptr[v + 1] - rb);  // < kSeg for non-big rows
-      }
-    }
+    // branch-free loads (a lane past n reads vertex 0): the per-vertex
+    // loads go out together, then the lane word through the slot
+    const bool inr = v < A.n;
+    const int32_t vv = inr ? v : 0;
+    const bool big = A.pin.is_big[vv] != 0;
+    const int32_t sv_me = A.slot_of[vv];
+    const int64_t rb0 = A.in_ptr[vv], re0 = A.in_ptr[vv + 1];
+    const uint64_t vw = A.vis[sv_me];
+    const uint64_t need = (inr && !big) ? pm & ~vw : 0ull;
+    const int64_t rb = rb0;
+    const int32_t len = need ? (int32_t)(re0 - rb0) : 0;  // < kSeg for non-big rows
     if (len) rows++;
     // exclusive prefix sum of row lengths -> virtual starts
     int32_t incl = len;
@@ -698,11 +748,13 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A, int d, uint64_t
     uint64_t got = 0;
     const int32_t my_end = vstart + len;
     // kUnroll steps per pass: their index loads, then their frontier-word
-    // gathers, are all in flight together; the scans then run step by step
-    // (a row already satisfied when the pass starts issues no loads).
+    // gathers, are all in flight together (no branches around the loads:
+    // idle lanes read entry 0 and the word of slot 0, both cached); the
+    // scans then run step by step.
     for (int32_t base0 = 0; base0 < total; base0 += kWave * kUnroll) {
       int rk[kUnroll];
       int32_t rsk[kUnroll], idx[kUnroll];
+      bool ok[kUnroll];
       uint64_t nrk[kUnroll], raw[kUnroll];
 #pragma unroll
       for (int k = 0; k < kUnroll; k++) {
@@ -721,22 +773,26 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A, int d, uint64_t
         const int64_t rbase = __shfl(rb, r);
         nrk[k] = shfl64(need, r);
         const uint64_t gr = shfl64(got, r);
-        idx[k] = -1;
-        if (ve < total && (nrk[k] & ~gr)) idx[k] = A.in_slot[rbase + (ve - rsk[k])];
+        ok[k] = ve < total && (nrk[k] & ~gr);
+        idx[k] = A.in_slot[ok[k] ? rbase + (ve - rsk[k]) : 0];
       }
-      // sparse frontier: frontier bits first (a small, cached array), then
-      // the frontier words of the entries whose bit is set
-      uint32_t fbw[kUnroll];
+      if (use_fb) {
+        // sparse frontier: the gather is needed only where the bit is set
 #pragma unroll
-      for (int k = 0; k < kUnroll; k++) fbw[k] = (use_fb && idx[k] >= 0) ? A.fb[idx[k] >> 5] : ~0u;
+        for (int k = 0; k < kUnroll; k++)
+          if (ok[k]) scanned++;
 #pragma unroll
-      for (int k = 0; k < kUnroll; k++) {
-        raw[k] = 0;
-        if (idx[k] >= 0) {
-          scanned++;
-          if ((fbw[k] >> (idx[k] & 31)) & 1u) raw[k] = fr[idx[k]] & nrk[k];
-        }
+        for (int k = 0; k < kUnroll; k++)
+          ok[k] = ok[k] && ((A.fb[idx[k] >> 5] >> (idx[k] & 31)) & 1u);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++)
+          if (ok[k]) scanned++;
       }
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) raw[k] = fr[ok[k] ? idx[k] : 0];
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) raw[k] = ok[k] ? raw[k] & nrk[k] : 0ull;
 #pragma unroll
       for (int k = 0; k < kUnroll; k++) {
         if (!__any(raw[k] != 0)) continue;  // no hit in this step: got unchanged
@@ -784,12 +840,16 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A, int d, uint64_t
         lb[k] = bits ? __ffsll((unsigned long long)bits) - 1 : -1;
         bits &= bits - 1;
       }
-#pragma unroll
-      for (int k = 0; k < kFlush; k++)
-        pv[k] = (lb[k] >= 0 && ((got >> lb[k]) & 1ull)) ? A.in_idx[rb + pt[lb[k]][lane]] : 0;
+      bool hb[kFlush];
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
-        if (lb[k] >= 0 && ((got >> lb[k]) & 1ull)) {
+        hb[k] = lb[k] >= 0 && ((got >> lb[k]) & 1ull);
+        const int off = pt[lb[k] >= 0 ? lb[k] : 0][lane];
+        pv[k] = A.in_idx[hb[k] ? rb + off : 0];  // no branch: loads go out together
+      }
+#pragma unroll
+      for (int k = 0; k < kFlush; k++) {
+        if (hb[k]) {
           int64_t o = (int64_t)lb[k] * A.n + v;
           A.parents[o] = pv[k];
           A.levels[o] = lev;
@@ -808,18 +868,16 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A, int d, uint64_t
     }
     if (tally_on) tally_words(got, clamp_deg(od), clamp_deg(len), sym, tally);
   }
-  // Big vertices: one active segment per wavefront.
+  // Big vertices: one active segment per wavefront; the record carries the
+  // lanes its vertex needs (vis of big vertices changes only in ms_pull_fin).
   const int64_t nact = (int64_t)ld_ctrl(&c->segtail[0]);
   for (int64_t i = wave; i < nact; i += nwaves) {
     const int32_t s = A.segq[0][i];
-    int32_t v = A.pin.seg_v[s];
-    uint64_t need = pm & ~A.vis[A.slot_of[v]];
-    if (!need) continue;
-    int32_t bid = A.pin.seg_bid[s];
-    int64_t beg = A.pin.seg_beg[s];
-    int64_t rend = A.in_ptr[v + 1];
-    int64_t end = beg + kSeg < rend ? beg + kSeg : rend;
-    uint64_t gj = ms_wave_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], use_fb, scanned);
+    const uint64_t need = A.segneed[i];
+    const int32_t bid = A.pin.seg_bid[s];
+    const int64_t beg = A.pin.seg_beg[s];
+    const int64_t end = beg + A.pin.seg_len[s];
+    uint64_t gj = ms_seg_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], use_fb, scanned);
     if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
   }
   if (tally_on) tally_flush(tally, A.lanes, tl);
@@ -923,9 +981,11 @@ __global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int d, int 
   for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; b0 < L.nbig; b0 += stride) {
     const int64_t b = b0 + lane;
     int32_t first = 0, cnt = 0;
+    uint64_t need = 0;
     if (b < L.nbig) {
       const int32_t sv = A.slot_of[L.big_v[b]];
-      const bool on = which ? (fr[sv] & mask) != 0 : (mask & ~A.vis[sv]) != 0;
+      need = mask & ~A.vis[sv];
+      const bool on = which ? (fr[sv] & mask) != 0 : need != 0;
       if (on) {
         first = L.seg_first[b];
         cnt = L.seg_first[b + 1] - first;
@@ -943,6 +1003,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int d, int 
     if (lane == 0) base = atomicAdd(tail, (unsigned long long)total);
     base = __shfl(base, 0) + (unsigned long long)(incl - cnt);
     for (int32_t k = 0; k < cnt; k++) out[base + k] = first + k;
+    if (!which)
+      for (int32_t k = 0; k < cnt; k++) A.segneed[base + k] = need;
   }
 }
 
@@ -1005,8 +1067,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int d, int phase, 
       const int32_t s = A.segq[1][it - nq];
       u = A.pout.seg_v[s];
       beg = A.pout.seg_beg[s];
-      int64_t rend = A.out_ptr[u + 1];
-      end = beg + kSeg < rend ? beg + kSeg : rend;
+      end = beg + A.pout.seg_len[s];
     }
     const uint64_t f = fr[A.slot_of[u]] & qm;
     if (!f) continue;
@@ -1215,6 +1276,31 @@ __global__ void ms_decide(MsArgs A, int d, MsParams P, int relist) {
   }
 }
 
+// Outputs of the vertices without in-edges: no root but the vertex itself
+// reaches them, so they are known before the search starts and are written
+// on a second stream while the first rounds run.  Plane-major: wavefront
+// item k is lane l = k / nch and vertices [64c, 64c + 64), c = k % nch, so
+// every output row is written front to back.
+__global__ void __launch_bounds__(kMsBlock) ms_fill_noin(int32_t *parents, int16_t *levels,
+                                                         const uint64_t *noin, int64_t n,
+                                                         int nroots, const int64_t *roots) {
+  const int lane = lane_id();
+  const int64_t nch = (n + kWave - 1) / kWave;
+  const int64_t total = nch * nroots;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  for (int64_t k = wave; k < total; k += nwaves) {
+    const int64_t l = k / nch, c = k - l * nch;
+    const uint64_t w = noin[c];
+    if (!w) continue;
+    const int64_t v = c * kWave + lane;  // a set bit implies v < n
+    if (((w >> lane) & 1ull) && v != roots[l]) {
+      parents[l * n + v] = -1;
+      levels[l * n + v] = -1;
+    }
+  }
+}
+
 // Final pass: -1 into every (lane, vertex) pair no lane word reached.  A
 // wavefront owns 64 consecutive vertices and walks the lane bits missing in
 // any of them, so each store instruction covers 64 consecutive entries of
@@ -1224,7 +1310,9 @@ __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < A.n; v0 += stride) {
     const int64_t v = v0 + lane;
-    const uint64_t miss = v < A.n ? A.act & ~A.vis[A.slot_of[v]] : 0ull;
+    // vertices without in-edges are written by ms_fill_noin
+    const bool skip = v >= A.n || ((A.noin[v0 >> 6] >> lane) & 1ull);
+    const uint64_t miss = skip ? 0ull : A.act & ~A.vis[A.slot_of[v]];
     uint64_t bits = wave_or64(miss);
     while (bits) {
       const int l = __ffsll((unsigned long long)bits) - 1;
@@ -1239,28 +1327,31 @@ __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
 }
 
 // GTEPS m per lane: sum of the per-vertex weight over each lane's reached
-// set.  Per wavefront of 64 vertices, one 64-bit wavefront sum per lane bit
-// present; lane l of the wavefront keeps lane bit l's total.
+// set.  A wavefront stages its 64 vertices' (word, weight) pairs in LDS and
+// lane l then walks them for lane bit l: a transpose through LDS with
+// broadcast reads, no cross-lane reductions.
 __global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const int32_t *slot_of,
                                                      const int32_t *w, int64_t n, int nroots,
                                                      unsigned long long *out) {
   __shared__ unsigned long long acc[64];
+  __shared__ uint64_t sb[kMsWaves][kWave];
+  __shared__ uint32_t sw[kMsWaves][kWave];
   if (threadIdx.x < 64) acc[threadIdx.x] = 0;
   __syncthreads();
-  const int lane = lane_id();
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
   unsigned long long mine = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < n; v0 += stride) {
     const int64_t v = v0 + lane;
-    const unsigned long long wv = v < n ? (unsigned long long)w[v] : 0ull;
-    const uint64_t b = wv ? vis[slot_of[v]] : 0ull;
-    uint64_t bits = wave_or64(b);
-    while (bits) {
-      const int l = __ffsll((unsigned long long)bits) - 1;
-      bits &= bits - 1;
-      const unsigned long long s = wave_sum_u64(((b >> l) & 1ull) ? wv : 0ull);
-      if (lane == l) mine += s;
-    }
+    const uint32_t wt = v < n ? (uint32_t)w[v] : 0u;
+    const uint64_t b = wt ? vis[slot_of[v]] : 0ull;
+    if (!__any(b != 0)) continue;
+    sb[wv][lane] = b;
+    sw[wv][lane] = wt;
+    wave_sync();
+    for (int r = 0; r < kWave; r++)
+      if ((sb[wv][r] >> lane) & 1ull) mine += sw[wv][r];
+    wave_sync();
   }
   if (mine) atomicAdd(&acc[lane], mine);
   __syncthreads();
@@ -1331,6 +1422,8 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   A.bgot = ws.bgot;
   A.segq[0] = ws.segq[0];
   A.segq[1] = ws.segq[1];
+  A.segneed = ws.segneed;
+  A.noin = md->noin;
   A.n = g->n;
   A.nroots = nroots;
   A.act = nroots == 64 ? ~0ull : ((1ull << nroots) - 1ull);
@@ -1343,7 +1436,9 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
     if (r == hipSuccess) evs.push_back(*e);
     return r;
   };
+  bool side_used = false;
   auto cleanup = [&]() {
+    if (side_used) (void)hipStreamSynchronize(g->side);  // the fill may still write outputs
     for (hipEvent_t e : evs) hipEventDestroy(e);
     if (tmp) hipFree(tmp);
   };
@@ -1352,6 +1447,18 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (e == hipSuccess) e = new_event(&e0);
   if (e == hipSuccess) e = new_event(&e1);
   if (e == hipSuccess) e = hipEventRecord(e0, st);
+  // The outputs of vertices without in-edges go out on the side stream,
+  // overlapping the first rounds; the stream joins before ms_finalize.
+  hipEvent_t side_done = nullptr;
+  if (e == hipSuccess && !g->side) e = hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = new_event(&side_done);
+  if (e == hipSuccess) e = hipStreamWaitEvent(g->side, e0, 0);
+  if (e == hipSuccess) {
+    ms_fill_noin<<<blocks, kMsBlock, 0, g->side>>>(dpar, dlev, md->noin, g->n, nroots, ws.roots);
+    side_used = true;
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(side_done, g->side);
   if (e != hipSuccess) {
     cleanup();
     return hip_fail(e, "msbfs setup", __FILE__, __LINE__);
@@ -1370,7 +1477,7 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   // One host poll per round: the round pulls the lanes in pmask, then pushes
   // the lanes in qmask; each part is bracketed by events.
-  struct RoundEvents { hipEvent_t a, p, b; };
+  struct RoundEvents { hipEvent_t a, k, p, b; };
   std::vector<RoundEvents> rev;
   int64_t d = 0;
   // listed: the current frontier's vertices are all listed in Q[d&1] (the
@@ -1385,8 +1492,8 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   };
   while (e == hipSuccess && !hc.done) {
     RoundEvents re;
-    if ((e = new_event(&re.a)) != hipSuccess || (e = new_event(&re.p)) != hipSuccess ||
-        (e = new_event(&re.b)) != hipSuccess)
+    if ((e = new_event(&re.a)) != hipSuccess || (e = new_event(&re.k)) != hipSuccess ||
+        (e = new_event(&re.p)) != hipSuccess || (e = new_event(&re.b)) != hipSuccess)
       break;
     const uint64_t pm = hc.pmask, qm = hc.qmask;
     hipEventRecord(re.a, st);
@@ -1395,10 +1502,12 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
       const int use_fb = hc.prev_nf * 8 <= g->n ? 1 : 0;
       if (use_fb) ms_mark<<<blocks, kMsBlock, 0, st>>>(A.fw[d & 1], g->n, A.fb);
       if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, (int)d, 0, pm);
+      hipEventRecord(re.k, st);
       if (tally) ms_pull<true><<<blocks, kMsBlock, 0, st>>>(A, (int)d, pm, use_fb);
       else ms_pull<false><<<blocks, kMsBlock, 0, st>>>(A, (int)d, pm, use_fb);
       if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, (int)d, tally);
     }
+    if (!pm) hipEventRecord(re.k, st);
     hipEventRecord(re.p, st);
     if (qm) {
       if (!listed) ms_compact<<<blocks, kMsBlock, 0, st>>>(A, (int)d, md->vertex_of);
@@ -1421,8 +1530,11 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   }
   // The final pass belongs to producing the outputs, so e1 follows it.
   if (e == hipSuccess) {
-    ms_finalize<<<blocks, kMsBlock, 0, st>>>(A);
-    e = hipGetLastError();
+    e = hipStreamWaitEvent(st, side_done, 0);
+    if (e == hipSuccess) {
+      ms_finalize<<<blocks, kMsBlock, 0, st>>>(A);
+      e = hipGetLastError();
+    }
   }
   if (e == hipSuccess) e = hipEventRecord(e1, st);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1483,7 +1595,7 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
         if (r < (int64_t)rev.size()) {
           float rm = 0, pms = 0;
           hipEventElapsedTime(&rm, rev[r].a, rev[r].b);
-          hipEventElapsedTime(&pms, rev[r].a, rev[r].p);
+          hipEventElapsedTime(&pms, rev[r].k, rev[r].p);
           if (stats->level_ms) stats->level_ms[r] = rm;
           if (stats->pull_ms) stats->pull_ms[r] = pms;
         }
@@ -1493,7 +1605,7 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (m_lanes) {
     unsigned long long m[64];
     ms_zero<<<1, 64, 0, st>>>(ws.mlane, 64);
-    ms_count<<<grid_blocks(g->device, 2), kMsBlock, 0, st>>>(ws.vis, md->slot_of, g->m_weight, g->n,
+    ms_count<<<grid_blocks(g->device, 8), kMsBlock, 0, st>>>(ws.vis, md->slot_of, g->m_weight, g->n,
                                                              nroots, ws.mlane);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(m, ws.mlane, sizeof(m), hipMemcpyDeviceToHost, st);

@@ -154,6 +154,28 @@ def test_msbfs_lanes_equal_oracle(scale, ef, seed, directed, nroots):
       _check_ms_stats(r.stats, oracle_levels, d)
 
 
+def test_msbfs_edge_cases():
+  """Roots without in-edges, isolated roots, the same root in several lanes,
+  a hub above the segment length, a deep path, n not a multiple of 64."""
+  n = 5000
+  e = [(i, i + 1) for i in range(1000)]                 # a path: 1000 levels
+  e += [(2000, j) for j in range(2001, 2700)]           # a hub of out-degree 699
+  e += [(j, 2000) for j in range(2700, 3100)]           # ... and in-degree 400
+  e += [(4000, 4001), (4001, 4002), (4002, 4000)]       # a directed cycle
+  e += [(4500, 4501)]                                   # 4500: out-edges only
+  g = _edges_graph(e, n)
+  roots = np.array([0, 4500, 4999, 2000, 2700, 4000, 0, 4999, 999, 3000, 4500])
+  levs = np.stack([O.bfs(g, int(s))[0] for s in roots])
+  with _device(g) as dg:
+    for d in DIRECTIONS + ["beamer_lanes"]:
+      r = dg.msbfs(roots, direction=d)
+      for l, s in enumerate(roots):
+        lev, par, _, _ = O.bfs(g, int(s))
+        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16), err_msg=f"lane {l} dir {d}")
+        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d}")
+      _check_ms_stats(r.stats, levs, d)
+
+
 def _check_ms_stats(st, levs, direction):
   """Round accounting of a multi-source pass against the lanes' oracle
   levels: round r reaches the (lane, vertex) pairs at level r+1, nf counts
