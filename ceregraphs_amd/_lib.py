@@ -49,6 +49,7 @@ class Options(ctypes.Structure):
       ("block", ctypes.c_int32),
       ("grid_mult", ctypes.c_int32),
       ("max_levels", ctypes.c_int64),
+      ("capture", ctypes.c_int32),
   ]
 
 

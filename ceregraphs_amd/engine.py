@@ -65,7 +65,7 @@ class MSBFSResult:
 
 
 def _options(direction="beamer", alpha=None, beta=None, tau=None, grid_mult=None,
-             max_levels=None, rounds_hint=None):
+             max_levels=None, rounds_hint=None, capture=False):
   o = _lib.Options()
   if isinstance(direction, str):
     direction = _lib.DIRECTIONS[direction]
@@ -76,6 +76,7 @@ def _options(direction="beamer", alpha=None, beta=None, tau=None, grid_mult=None
   o.grid_mult = int(grid_mult or 0)
   o.max_levels = int(max_levels or 0)
   o.rounds_hint = int(rounds_hint or 0)
+  o.capture = 1 if capture else 0
   return o
 
 

@@ -65,6 +65,9 @@ typedef struct {
   int32_t block;      /* threads per block for level kernels          (256)  */
   int32_t grid_mult;  /* blocks per CU for grid-stride kernels        (8)    */
   int64_t max_levels; /* abort with CGBFS_ETIMEOUT beyond this        (n+1)  */
+  int32_t capture;    /* multi-source: capture the round kernels once into a
+                         HIP graph and replay it, deciding every level on the
+                         device (no per-round host poll, no per-round events) */
 } cgbfs_options;
 
 /* Per-search statistics.  The caller owns the history arrays (capacity

@@ -144,12 +144,14 @@ def test_msbfs_lanes_equal_oracle(scale, ef, seed, directed, nroots):
     roots[1] = roots[2]  # a repeated root
   oracle_levels = np.stack([O.bfs(g, int(s))[0] for s in roots])
   with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
-    for d in DIRECTIONS + ["beamer_lanes"]:
-      r = dg.msbfs(roots, direction=d)
+    for d, cap in [(d, False) for d in DIRECTIONS + ["beamer_lanes"]] + [("beamer", True),
+                                                                       ("beamer_lanes", True)]:
+      r = dg.msbfs(roots, direction=d, capture=cap)
       for l, s in enumerate(roots):
         lev, par, _, rounds = O.bfs(g, int(s))
-        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16), err_msg=f"lane {l} dir {d}")
-        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d}")
+        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16),
+                                      err_msg=f"lane {l} dir {d} capture {cap}")
+        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d} capture {cap}")
         assert r.m_lanes[l] == O.m_edges(g, lev)
       _check_ms_stats(r.stats, oracle_levels, d)
 
@@ -167,12 +169,14 @@ def test_msbfs_edge_cases():
   roots = np.array([0, 4500, 4999, 2000, 2700, 4000, 0, 4999, 999, 3000, 4500])
   levs = np.stack([O.bfs(g, int(s))[0] for s in roots])
   with _device(g) as dg:
-    for d in DIRECTIONS + ["beamer_lanes"]:
-      r = dg.msbfs(roots, direction=d)
+    for d, cap in [(d, False) for d in DIRECTIONS + ["beamer_lanes"]] + [("beamer", True),
+                                                                       ("tau", True)]:
+      r = dg.msbfs(roots, direction=d, capture=cap)
       for l, s in enumerate(roots):
         lev, par, _, _ = O.bfs(g, int(s))
-        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16), err_msg=f"lane {l} dir {d}")
-        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d}")
+        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16),
+                                      err_msg=f"lane {l} dir {d} capture {cap}")
+        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d} capture {cap}")
       _check_ms_stats(r.stats, levs, d)
 
 
