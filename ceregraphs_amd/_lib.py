@@ -29,7 +29,8 @@ DIRECTIONS = {"beamer": DIR_BEAMER, "topdown": DIR_TOPDOWN, "bottomup": DIR_BOTT
 EXPORTED = (
     "cgbfs_last_error", "cgbfs_version", "cgbfs_device_count", "cgbfs_graph_create",
     "cgbfs_graph_create_rmat", "cgbfs_graph_info", "cgbfs_graph_download",
-    "cgbfs_graph_out_degree", "cgbfs_graph_destroy", "cgbfs_bfs", "cgbfs_msbfs", "cgbfs_sync",
+    "cgbfs_graph_out_degree", "cgbfs_graph_destroy", "cgbfs_bfs", "cgbfs_msbfs",
+    "cgbfs_msbfs_seeded", "cgbfs_sync",
 )
 
 
@@ -100,6 +101,9 @@ _SIGNATURES = {
                                  ctypes.POINTER(Options), ctypes.POINTER(Stats)]),
     "cgbfs_msbfs": (ctypes.c_int, [_GP, _P, ctypes.c_int32, _P, _P, ctypes.c_int32,
                                    ctypes.POINTER(Options), ctypes.POINTER(Stats), _P]),
+    "cgbfs_msbfs_seeded": (ctypes.c_int, [_GP, _P, _P, ctypes.c_int64, ctypes.c_int32, _P, _P,
+                                          ctypes.c_int32, ctypes.POINTER(Options),
+                                          ctypes.POINTER(Stats), _P]),
     "cgbfs_sync": (ctypes.c_int, [_GP]),
 }
 

@@ -192,6 +192,8 @@ struct MsGraphData {
   // arguments it was captured with; reused while they stay the same.
   hipGraphExec_t exec = nullptr;
   std::vector<char> exec_key;
+  void *seedbuf = nullptr;  // the call's seed vertices and lanes
+  int64_t seedcap = 0;
 };
 
 struct MsWorkspace {
@@ -202,7 +204,6 @@ struct MsWorkspace {
   MsHist *hist;
   MsLane *lanes;
   unsigned long long *mlane;
-  int64_t *roots;
   uint32_t *cand;
   uint64_t *bgot;
   int32_t *segq[2];
@@ -364,6 +365,7 @@ static int build_slots(DeviceGraph *g, MsGraphData *md) {
 static void free_msdata(void *q) {
   MsGraphData *m = (MsGraphData *)q;
   if (m->exec) hipGraphExecDestroy(m->exec);
+  if (m->seedbuf) hipFree(m->seedbuf);
   if (m->mem) hipFree(m->mem);
   if (m->noin) hipFree(m->noin);
   if (m->out_slot && m->out_slot != m->in_slot) hipFree(m->out_slot);
@@ -392,7 +394,7 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
   size_t wb = al(sizeof(uint64_t) * n), qb = al(sizeof(int32_t) * n);
   size_t fbb = al(sizeof(uint64_t) * ((n + 63) / 64));  // whole 64-bit words per wavefront
   size_t total = 3 * wb + fbb + 2 * qb + al(sizeof(MsCtrl)) + al(sizeof(MsHist)) + al(sizeof(MsLane)) +
-                 al(sizeof(unsigned long long) * 64) + al(sizeof(int64_t) * 64) +
+                 al(sizeof(unsigned long long) * 64) +
                  al(sizeof(uint32_t) * 64 * (nbig + 1)) + al(sizeof(uint64_t) * (nbig + 1)) +
                  al(sizeof(int32_t) * (md->pin.nseg + 1)) + al(sizeof(int32_t) * (md->pout.nseg + 1)) +
                  al(sizeof(uint64_t) * (md->pin.nseg + 1));
@@ -414,7 +416,6 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
   w->hist = (MsHist *)p; p += al(sizeof(MsHist));
   w->lanes = (MsLane *)p; p += al(sizeof(MsLane));
   w->mlane = (unsigned long long *)p; p += al(sizeof(unsigned long long) * 64);
-  w->roots = (int64_t *)p; p += al(sizeof(int64_t) * 64);
   w->cand = (uint32_t *)p; p += al(sizeof(uint32_t) * 64 * (nbig + 1));
   w->bgot = (uint64_t *)p; p += al(sizeof(uint64_t) * (nbig + 1));
   w->segq[0] = (int32_t *)p; p += al(sizeof(int32_t) * (md->pin.nseg + 1));
@@ -633,26 +634,38 @@ __host__ __device__ inline bool lane_policy(int mode) {
   return mode == CGBFS_DIR_TAU || mode == CGBFS_DIR_BEAMER_LANES;
 }
 
-// One wavefront: thread 0 seeds the roots; thread l sets lane l's direction
-// state the way the single-root search starts (round 0 pushes unless every
-// round pulls, bool_pe.csl:714-719).
-__global__ void ms_init_roots(MsArgs A, const int64_t *roots, MsParams P) {
+// One wavefront.  Thread 0 plants the seeds (vertex sv[k] starts lane
+// sl[k]) in the lane words and lists their distinct vertices; thread l sets
+// lane l's direction state the way the single-root search starts (round 0
+// pushes unless every round pulls, bool_pe.csl:714-719).  The seeds' own
+// outputs are written by ms_seed_outputs after the output fill.
+__global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, int64_t nseeds,
+                              MsParams P) {
   if (blockIdx.x != 0) return;
   MsCtrl *c = A.ctrl;
   MsLane *L = A.lanes;
   const int l = threadIdx.x;
+  __shared__ unsigned long long s_nq, s_deg;
   if (l == 0) {
-    unsigned long long nq = 0;
-    for (int k = 0; k < A.nroots; k++) {
-      int64_t r = roots[k];
-      int32_t sr = A.slot_of[r];
-      uint64_t bit = 1ull << k;
-      if (A.fw[0][sr] == 0) A.q[0][nq++] = (int32_t)r;
-      A.vis[sr] |= bit;
-      A.fw[0][sr] |= bit;
-      A.parents[(int64_t)k * A.n + r] = (int32_t)r;
-      A.levels[(int64_t)k * A.n + r] = 0;
+    s_nq = 0;
+    s_deg = 0;
+  }
+  __syncthreads();
+  // the first lane bit planted on a vertex lists it (and counts its edges)
+  for (int64_t k = l; k < nseeds; k += blockDim.x) {
+    const int64_t r = sv[k];
+    const int32_t sr = A.slot_of[r];
+    const unsigned long long bit = 1ull << sl[k];
+    atomicOr((unsigned long long *)&A.vis[sr], bit);
+    if (atomicOr((unsigned long long *)&A.fw[0][sr], bit) == 0) {
+      A.q[0][atomicAdd(&s_nq, 1ull)] = (int32_t)r;
+      atomicAdd(&s_deg, (unsigned long long)(A.out_ptr[r + 1] - A.out_ptr[r]));
     }
+  }
+  __syncthreads();
+  const unsigned long long deg = s_deg;  // out-degree sum of the distinct seed vertices
+  if (l == 0) {
+    const unsigned long long nq = s_nq;
     st_ctrl(&c->qtail[0], nq);
     st_ctrl(&c->qtail[1], 0ull);
     st_ctrl(&c->nf, 0ull);
@@ -669,27 +682,24 @@ __global__ void ms_init_roots(MsArgs A, const int64_t *roots, MsParams P) {
   }
   bool live = false;
   int dir = 0;
-  unsigned long long deg = 0;  // out-degree of lane l's root, first lane of each root only
   if (l < 64) {
     st_ctrl(&L->cnt[l], 0ull);
     st_ctrl(&L->out[l], 0ull);
     st_ctrl(&L->in[l], 0ull);
     if (l < A.nroots) {
-      const int64_t r = roots[l];
-      const long long mu = (long long)P.nnz - (long long)(A.in_ptr[r + 1] - A.in_ptr[r]);
+      long long mu = (long long)P.nnz;
+      for (int64_t k = 0; k < nseeds; k++)
+        if (sl[k] == l) mu -= (long long)(A.in_ptr[sv[k] + 1] - A.in_ptr[sv[k]]);
       if (P.mode == CGBFS_DIR_BOTTOMUP) dir = 1;
       st_ctrl(&L->mu[l], mu);
       st_ctrl(&L->prev[l], 1ll);
       live = true;
-      bool first = true;
-      for (int k = 0; k < l; k++) first = first && roots[k] != r;
-      if (first) deg = (unsigned long long)(A.out_ptr[r + 1] - A.out_ptr[r]);
     }
     st_ctrl(&L->dir[l], dir);
   }
   // the union-count policy starts like the per-lane rules, except that 64
   // roots may already hold enough edges to pull from round 0
-  const unsigned long long mf = wave_sum_u64(deg);
+  const unsigned long long mf = deg;
   int g = 0;
   if (P.mode == CGBFS_DIR_BOTTOMUP) g = 1;
   else if (P.mode == CGBFS_DIR_BEAMER && (double)mf * P.alpha > (double)P.nnz) g = 1;
@@ -1351,13 +1361,14 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
 }
 
 // Outputs of the vertices without in-edges: no root but the vertex itself
-// reaches them, so they are known before the search starts and are written
-// on a second stream while the first rounds run.  Plane-major: wavefront
+// reaches them, so they are known before the search starts (-1; a seed's own
+// entry is rewritten by ms_seed_outputs afterwards) and are written on a
+// second stream while the first rounds run.  Plane-major: wavefront
 // item k is lane l = k / nch and vertices [64c, 64c + 64), c = k % nch, so
 // every output row is written front to back.
 __global__ void __launch_bounds__(kMsBlock) ms_fill_noin(int32_t *parents, int16_t *levels,
                                                          const uint64_t *noin, int64_t n,
-                                                         int nroots, const int64_t *roots) {
+                                                         int nroots) {
   const int lane = lane_id();
   const int64_t nch = (n + kWave - 1) / kWave;
   const int64_t total = nch * nroots;
@@ -1368,11 +1379,20 @@ __global__ void __launch_bounds__(kMsBlock) ms_fill_noin(int32_t *parents, int16
     const uint64_t w = noin[c];
     if (!w) continue;
     const int64_t v = c * kWave + lane;  // a set bit implies v < n
-    if (((w >> lane) & 1ull) && v != roots[l]) {
+    if ((w >> lane) & 1ull) {
       parents[l * n + v] = -1;
       levels[l * n + v] = -1;
     }
   }
+}
+
+// The seeds' own outputs (parent = itself, level 0), after the output fill.
+__global__ void ms_seed_outputs(MsArgs A, const int64_t *sv, const int32_t *sl, int64_t nseeds) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nseeds) return;
+  const int64_t o = (int64_t)sl[k] * A.n + sv[k];
+  A.parents[o] = (int32_t)sv[k];
+  A.levels[o] = 0;
 }
 
 // Final pass: -1 into every (lane, vertex) pair no lane word reached.  A
@@ -1440,15 +1460,12 @@ __global__ void ms_zero(unsigned long long *p, int k) {
 
 using namespace cg;
 
-extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots, int32_t *parents,
-                           int16_t *levels, int32_t on_device, const cgbfs_options *opt,
-                           cgbfs_stats *stats, int64_t *m_lanes) {
-  if (!h) return set_error(CGBFS_EINVAL, "graph is null");
+// The multi-source search behind both entry points: nseeds seeds (vertex
+// sv[k] starts lane sl[k]) over nroots lanes.
+static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t nseeds,
+                  int32_t nroots, int32_t *parents, int16_t *levels, int32_t on_device,
+                  const cgbfs_options *opt, cgbfs_stats *stats, int64_t *m_lanes) {
   DeviceGraph *g = &h->g;
-  if (!roots || nroots < 1 || nroots > 64) return set_error(CGBFS_EINVAL, "nroots must be in [1, 64]");
-  if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
-  for (int l = 0; l < nroots; l++)
-    if (roots[l] < 0 || roots[l] >= g->n) return set_error(CGBFS_EINVAL, "root out of range");
   cgbfs_options o;
   fill_options(opt, g->n, &o);
   if (o.direction < 0 || o.direction > CGBFS_DIR_BEAMER_LANES)
@@ -1460,6 +1477,15 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   int rc = ensure_wsm(g, &ws, &md);
   if (rc) return rc;
   hipStream_t st = g->stream;
+  if (md->seedcap < nseeds) {
+    if (md->seedbuf) hipFree(md->seedbuf);
+    md->seedbuf = nullptr;
+    md->seedcap = 0;
+    CG_HIP(hipMalloc(&md->seedbuf, al(sizeof(int64_t) * nseeds) + sizeof(int32_t) * nseeds));
+    md->seedcap = nseeds;
+  }
+  int64_t *dsv = (int64_t *)md->seedbuf;
+  int32_t *dsl = (int32_t *)((char *)md->seedbuf + al(sizeof(int64_t) * nseeds));
 
   const size_t plane = (size_t)g->n * nroots;
   int32_t *dpar = parents;
@@ -1519,7 +1545,8 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
     if (tmp) hipFree(tmp);
   };
   hipEvent_t e0, e1;
-  hipError_t e = hipMemcpyAsync(ws.roots, roots, sizeof(int64_t) * nroots, hipMemcpyHostToDevice, st);
+  hipError_t e = hipMemcpyAsync(dsv, sv, sizeof(int64_t) * nseeds, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dsl, sl, sizeof(int32_t) * nseeds, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = new_event(&e0);
   if (e == hipSuccess) e = new_event(&e1);
   if (e == hipSuccess) e = hipEventRecord(e0, st);
@@ -1530,7 +1557,7 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (e == hipSuccess) e = new_event(&side_done);
   if (e == hipSuccess) e = hipStreamWaitEvent(g->side, e0, 0);
   if (e == hipSuccess) {
-    ms_fill_noin<<<blocks, kMsBlock, 0, g->side>>>(dpar, dlev, md->noin, g->n, nroots, ws.roots);
+    ms_fill_noin<<<blocks, kMsBlock, 0, g->side>>>(dpar, dlev, md->noin, g->n, nroots);
     side_used = true;
     e = hipGetLastError();
   }
@@ -1548,7 +1575,7 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   P.tau = (long long)o.tau;
   P.max_levels = (long long)o.max_levels;
   P.nnz = (long long)g->nnz;
-  ms_init_roots<<<1, 64, 0, st>>>(A, ws.roots, P);
+  ms_init_seeds<<<1, 64, 0, st>>>(A, dsv, dsl, nseeds, P);
   MsCtrl hc;
   std::memset(&hc, 0, sizeof(hc));
   struct RoundEvents { hipEvent_t a, k, p, b; };
@@ -1652,6 +1679,7 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   if (e == hipSuccess) {
     e = hipStreamWaitEvent(st, side_done, 0);
     if (e == hipSuccess) {
+      ms_seed_outputs<<<(unsigned)((nseeds + 255) / 256), 256, 0, st>>>(A, dsv, dsl, nseeds);
       ms_finalize<<<blocks, kMsBlock, 0, st>>>(A);
       e = hipGetLastError();
     }
@@ -1738,4 +1766,32 @@ extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots,
   }
   cleanup();
   return CGBFS_OK;
+}
+
+extern "C" int cgbfs_msbfs(cgbfs_graph *h, const int64_t *roots, int32_t nroots, int32_t *parents,
+                           int16_t *levels, int32_t on_device, const cgbfs_options *opt,
+                           cgbfs_stats *stats, int64_t *m_lanes) {
+  if (!h) return set_error(CGBFS_EINVAL, "graph is null");
+  if (!roots || nroots < 1 || nroots > 64) return set_error(CGBFS_EINVAL, "nroots must be in [1, 64]");
+  if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
+  for (int l = 0; l < nroots; l++)
+    if (roots[l] < 0 || roots[l] >= h->g.n) return set_error(CGBFS_EINVAL, "root out of range");
+  int32_t lanes[64];
+  for (int l = 0; l < nroots; l++) lanes[l] = l;
+  return ms_run(h, roots, lanes, nroots, nroots, parents, levels, on_device, opt, stats, m_lanes);
+}
+
+extern "C" int cgbfs_msbfs_seeded(cgbfs_graph *h, const int64_t *verts, const int32_t *lanes,
+                                  int64_t nseeds, int32_t nlanes, int32_t *parents, int16_t *levels,
+                                  int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats,
+                                  int64_t *m_lanes) {
+  if (!h) return set_error(CGBFS_EINVAL, "graph is null");
+  if (nlanes < 1 || nlanes > 64) return set_error(CGBFS_EINVAL, "nlanes must be in [1, 64]");
+  if (!verts || !lanes || nseeds < 1) return set_error(CGBFS_EINVAL, "need at least one seed");
+  if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
+  for (int64_t k = 0; k < nseeds; k++) {
+    if (verts[k] < 0 || verts[k] >= h->g.n) return set_error(CGBFS_EINVAL, "seed vertex out of range");
+    if (lanes[k] < 0 || lanes[k] >= nlanes) return set_error(CGBFS_EINVAL, "seed lane out of range");
+  }
+  return ms_run(h, verts, lanes, nseeds, nlanes, parents, levels, on_device, opt, stats, m_lanes);
 }

@@ -248,6 +248,36 @@ class DeviceGraph:
     return MSBFSResult(roots, par, lev, m, sb.result())
 
 
+  def msbfs_seeded(self, verts, lanes, nlanes, direction="beamer", device_out=None, **kw):
+    """Multi-source pass from seeds: seed k starts lane lanes[k] at vertex
+    verts[k]; a lane may hold several seeds (see cgbfs_msbfs_seeded)."""
+    verts = np.ascontiguousarray(verts, dtype=np.int64)
+    lanes = np.ascontiguousarray(lanes, dtype=np.int32)
+    if len(verts) != len(lanes) or not len(verts):
+      raise ValueError("verts and lanes must be non-empty and of equal length")
+    if not 1 <= nlanes <= 64:
+      raise ValueError("nlanes must be in [1, 64]")
+    sb = _StatsBuf()
+    opt = _options(direction, **kw)
+    m = np.zeros(nlanes, np.int64)
+    if device_out is not None:
+      par, lev = device_out
+      _check_tensor(par, nlanes * self.n, "int32")
+      _check_tensor(lev, nlanes * self.n, "int16")
+      rc = _lib.load().cgbfs_msbfs_seeded(self._h, _ptr(verts), _ptr(lanes), len(verts), nlanes,
+                                          ctypes.c_void_p(par.data_ptr()),
+                                          ctypes.c_void_p(lev.data_ptr()), 1, ctypes.byref(opt),
+                                          ctypes.byref(sb.s), _ptr(m))
+    else:
+      par = np.empty((nlanes, self.n), np.int32)
+      lev = np.empty((nlanes, self.n), np.int16)
+      rc = _lib.load().cgbfs_msbfs_seeded(self._h, _ptr(verts), _ptr(lanes), len(verts), nlanes,
+                                          _ptr(par), _ptr(lev), 0, ctypes.byref(opt),
+                                          ctypes.byref(sb.s), _ptr(m))
+    _lib.check(rc)
+    return MSBFSResult(verts, par, lev, m, sb.result())
+
+
 def _check_tensor(t, numel, dtype):
   import torch  # plumbing only: device memory for outputs that stay in HBM
   want = {"int32": torch.int32, "int16": torch.int16}[dtype]

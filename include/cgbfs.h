@@ -163,6 +163,20 @@ CGBFS_API int cgbfs_msbfs(cgbfs_graph *g, const int64_t *roots, int32_t nroots, 
                 int16_t *levels, int32_t on_device, const cgbfs_options *opt,
                 cgbfs_stats *stats, int64_t *m_lanes);
 
+/* Multi-source BFS from seeds: seed k starts lane lanes[k] at vertex
+ * verts[k] (level 0, parent itself); nlanes <= 64 lanes, any number of
+ * seeds per lane.  A lane's levels are distances to its nearest seed.  Seeds
+ * of one lane placed in components that no path joins give each component
+ * its own single-root search: this is how an uneven batch of several graphs
+ * (laid side by side as one disjoint union) runs in one pass, root l of
+ * every graph in lane l.  Outputs, stats and m_lanes as for cgbfs_msbfs
+ * (m_lanes sums over all of a lane's seeds).  No counterpart in the
+ * reference, which runs one search per launch (run_graph500.py:246-375). */
+CGBFS_API int cgbfs_msbfs_seeded(cgbfs_graph *g, const int64_t *verts, const int32_t *lanes,
+                       int64_t nseeds, int32_t nlanes, int32_t *parents, int16_t *levels,
+                       int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats,
+                       int64_t *m_lanes);
+
 /* Blocks until all work queued on the graph's stream has finished. */
 CGBFS_API int cgbfs_sync(cgbfs_graph *g);
 

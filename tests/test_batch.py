@@ -1,0 +1,88 @@
+"""Uneven multi-graph batches (ceregraphs_amd/batch.py).
+
+CPU: the disjoint union is built right (offsets, both views, orientation)
+and the property the batch rests on holds on the oracle -- a search from
+graph g's root inside the union reaches exactly graph g's single-root
+levels and parents, and nothing outside graph g.
+GPU: every (graph, root) search of a batch pass is bit-exact against the
+oracle on its own graph, directed and undirected, in both round drivers."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ceregraphs_amd import batch as B
+from ceregraphs_amd import gen_rmat
+
+SMALL_DIRECTED = [(8, 8, 10), (10, 32, 11), (9, 16, 12), (11, 8, 13)]
+
+
+def _parts(specs, directed=True):
+  abcd = B.DIRECTED_ABCD if directed else gen_rmat.GRAPH500_ABCD
+  out = []
+  for s, ef, seed in specs:
+    n, ip, ii, op, oi = gen_rmat.rmat_csr(s, ef, seed, *abcd[:3], directed=directed)
+    out.append((n, ip, ii, op, oi))
+  return out
+
+
+def test_union_csr_offsets_and_views():
+  parts = _parts(SMALL_DIRECTED)
+  n, ip, ii, op, oi, offs = B.union_csr(parts)
+  assert n == sum(p[0] for p in parts) and offs[-1] == n
+  for g, (gn, gip, gii, gop, goi) in enumerate(parts):
+    lo, hi = offs[g], offs[g + 1]
+    np.testing.assert_array_equal(np.diff(ip[lo:hi + 1]), np.diff(gip))
+    np.testing.assert_array_equal(ii[ip[lo]:ip[hi]], gii + lo)
+    np.testing.assert_array_equal(oi[op[lo]:op[hi]], goi + lo)
+  # rows stay ascending and no edge leaves its graph
+  for g in range(len(parts)):
+    seg = ii[ip[offs[g]]:ip[offs[g + 1]]]
+    assert seg.min() >= offs[g] and seg.max() < offs[g + 1]
+
+
+def test_search_inside_union_equals_search_on_the_graph():
+  parts = _parts(SMALL_DIRECTED)
+  n, ip, ii, op, oi, offs = B.union_csr(parts)
+  u = O.Graph(n, ip, ii, op, oi, False)
+  roots = B.pick_roots([np.diff(p[3]) for p in parts], 3, 18)
+  for g, p in enumerate(parts):
+    gg = O.Graph(p[0], p[1], p[2], p[3], p[4], False)
+    for r in roots[g]:
+      lev, par, _, _ = O.bfs(gg, int(r))
+      ulev, upar, _, _ = O.bfs(u, int(r) + int(offs[g]))
+      lo, hi = offs[g], offs[g + 1]
+      np.testing.assert_array_equal(ulev[lo:hi], lev)
+      np.testing.assert_array_equal(np.where(upar[lo:hi] >= 0, upar[lo:hi] - lo, -1), par)
+      assert (ulev[:lo] == -1).all() and (ulev[hi:] == -1).all()
+
+
+def test_pick_roots_rule():
+  degs = [np.array([0, 2, 1, 0, 5]), np.array([1, 1, 0])]
+  roots = B.pick_roots(degs, 2, 18)
+  for d, r in zip(degs, roots):
+    assert len(set(r.tolist())) == 2 and (d[r] > 0).all()
+  again = B.pick_roots(degs, 2, 18)
+  for a, b in zip(roots, again):
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("directed,specs,per_graph", [
+    (True, SMALL_DIRECTED, 32), (False, [(9, 16, 21), (7, 8, 22), (10, 16, 23)], 20)])
+def test_batch_searches_equal_oracle(directed, specs, per_graph):
+  parts = _parts(specs, directed)
+  with B.UnevenBatch(parts, symmetric=not directed) as batch:
+    roots = B.pick_roots(batch.graph_out_degrees(), per_graph, 18)
+    for capture, cfg in [(True, {}), (False, {}), (True, {"direction": "beamer_lanes"}),
+                         (True, {"direction": "bottomup"})]:
+      res = batch.run(roots, capture=capture, **cfg)
+      levels, parents = np.asarray(res.levels), np.asarray(res.parents)
+      m = batch.m_table(levels)
+      for g, p in enumerate(parts):
+        gg = O.Graph(p[0], p[1], p[2], p[3], p[4], not directed)
+        for l, r in enumerate(roots[g]):
+          lev, par, _, _ = O.bfs(gg, int(r))
+          blev, bpar = batch.instance(res, g, l, levels, parents)
+          np.testing.assert_array_equal(blev, lev, err_msg=f"graph {g} root {l} {cfg} {capture}")
+          np.testing.assert_array_equal(bpar, par, err_msg=f"graph {g} root {l} {cfg} {capture}")
+          assert m[g, l] == O.m_edges(gg, lev)
