@@ -30,7 +30,8 @@ EXPORTED = (
     "cgbfs_last_error", "cgbfs_version", "cgbfs_device_count", "cgbfs_graph_create",
     "cgbfs_graph_create_rmat", "cgbfs_graph_info", "cgbfs_graph_download",
     "cgbfs_graph_out_degree", "cgbfs_graph_destroy", "cgbfs_bfs", "cgbfs_msbfs",
-    "cgbfs_msbfs_seeded", "cgbfs_sync",
+    "cgbfs_msbfs_seeded", "cgbfs_sync", "cgbfs_engine_create", "cgbfs_engine_submit",
+    "cgbfs_engine_step", "cgbfs_engine_poll", "cgbfs_engine_destroy",
 )
 
 
@@ -105,6 +106,11 @@ _SIGNATURES = {
                                           ctypes.c_int32, ctypes.POINTER(Options),
                                           ctypes.POINTER(Stats), _P]),
     "cgbfs_sync": (ctypes.c_int, [_GP]),
+    "cgbfs_engine_create": (ctypes.c_int, [_GP, ctypes.POINTER(Options), ctypes.POINTER(_GP)]),
+    "cgbfs_engine_submit": (ctypes.c_int, [_GP, ctypes.c_int32, ctypes.c_int64, _P, _P]),
+    "cgbfs_engine_step": (ctypes.c_int, [_GP]),
+    "cgbfs_engine_poll": (ctypes.c_int, [_GP, ctypes.c_int32, _I32P, _U64P, _U64P, _P, _P]),
+    "cgbfs_engine_destroy": (ctypes.c_int, [_GP]),
 }
 
 _lock = threading.Lock()

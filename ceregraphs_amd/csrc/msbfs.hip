@@ -108,6 +108,17 @@ struct MsLane {
 };
 constexpr uint32_t kDegClamp = 1u << 25;  // 64 clamped degrees sum below 2^32
 
+// Where lane l's outputs go: parents par[l][v] (int32) and levels lev[l][v]
+// (int16), the level of a vertex reached in round d being d + 1 - base[l].
+// A one-call search points lane l at row l of [nroots][n] arrays with base
+// 0; a serving engine points each lane at its current query's own buffers
+// and sets base to the round the query was seeded in.
+struct LaneOut {
+  int32_t *par[64];
+  int16_t *lev[64];
+  int32_t base[64];
+};
+
 // Per-search constants, passed by value to the kernels that need them.
 struct MsParams {
   int mode;
@@ -143,8 +154,7 @@ struct MsArgs {
   const int32_t *out_idx;
   const int64_t *in_ptr;
   const int32_t *in_idx;
-  int32_t *parents;  // [nroots][n]
-  int16_t *levels;   // [nroots][n]
+  const LaneOut *out;  // per-lane output rows and level bases
   uint64_t *vis;
   uint64_t *fw[2];   // frontier words, rotated by level parity
   // One bit per slot: the current frontier word is nonzero.  Rebuilt before
@@ -203,6 +213,7 @@ struct MsWorkspace {
   MsCtrl *ctrl;
   MsHist *hist;
   MsLane *lanes;
+  LaneOut *out;
   unsigned long long *mlane;
   uint32_t *cand;
   uint64_t *bgot;
@@ -375,8 +386,45 @@ static void free_msdata(void *q) {
   delete m;
 }
 
-static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
-  const int64_t n = g->n;
+// Lays a multi-source workspace out from `base` (sizes only when base is
+// null) and returns its total size.
+static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspace *w) {
+  const int64_t nbig = md->pin.nbig;
+  const size_t wb = al(sizeof(uint64_t) * n), qb = al(sizeof(int32_t) * n);
+  const size_t fbb = al(sizeof(uint64_t) * ((n + 63) / 64));  // whole 64-bit words per wavefront
+  const size_t sizes[] = {
+      wb, wb, wb, fbb, qb, qb, al(sizeof(MsCtrl)), al(sizeof(MsHist)), al(sizeof(MsLane)),
+      al(sizeof(LaneOut)), al(sizeof(unsigned long long) * 64),
+      al(sizeof(uint32_t) * 64 * (nbig + 1)), al(sizeof(uint64_t) * (nbig + 1)),
+      al(sizeof(int32_t) * (md->pin.nseg + 1)), al(sizeof(int32_t) * (md->pout.nseg + 1)),
+      al(sizeof(uint64_t) * (md->pin.nseg + 1))};
+  size_t total = 0;
+  for (size_t x : sizes) total += x;
+  if (!base) return total;
+  char *p = base;
+  int i = 0;
+  auto take = [&]() { char *q = p; p += sizes[i++]; return q; };
+  w->vis = (uint64_t *)take();
+  w->fw[0] = (uint64_t *)take();
+  w->fw[1] = (uint64_t *)take();
+  w->fb = (uint32_t *)take();
+  w->q[0] = (int32_t *)take();
+  w->q[1] = (int32_t *)take();
+  w->ctrl = (MsCtrl *)take();
+  w->hist = (MsHist *)take();
+  w->lanes = (MsLane *)take();
+  w->out = (LaneOut *)take();
+  w->mlane = (unsigned long long *)take();
+  w->cand = (uint32_t *)take();
+  w->bgot = (uint64_t *)take();
+  w->segq[0] = (int32_t *)take();
+  w->segq[1] = (int32_t *)take();
+  w->segneed = (uint64_t *)take();
+  return total;
+}
+
+// The per-graph multi-source data (segment lists, slots), built on first use.
+static int ensure_msdata(DeviceGraph *g, MsGraphData **mdp) {
   if (!g->msdata) {
     MsGraphData *md = new MsGraphData();
     int rc = build_seglists(g, md);
@@ -388,16 +436,14 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
     g->msdata = md;
     g->msdata_free = free_msdata;
   }
-  MsGraphData *md = (MsGraphData *)g->msdata;
-  *mdp = md;
-  const int64_t nbig = md->pin.nbig;
-  size_t wb = al(sizeof(uint64_t) * n), qb = al(sizeof(int32_t) * n);
-  size_t fbb = al(sizeof(uint64_t) * ((n + 63) / 64));  // whole 64-bit words per wavefront
-  size_t total = 3 * wb + fbb + 2 * qb + al(sizeof(MsCtrl)) + al(sizeof(MsHist)) + al(sizeof(MsLane)) +
-                 al(sizeof(unsigned long long) * 64) +
-                 al(sizeof(uint32_t) * 64 * (nbig + 1)) + al(sizeof(uint64_t) * (nbig + 1)) +
-                 al(sizeof(int32_t) * (md->pin.nseg + 1)) + al(sizeof(int32_t) * (md->pout.nseg + 1)) +
-                 al(sizeof(uint64_t) * (md->pin.nseg + 1));
+  *mdp = (MsGraphData *)g->msdata;
+  return CGBFS_OK;
+}
+
+static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
+  int rc = ensure_msdata(g, mdp);
+  if (rc) return rc;
+  const size_t total = ws_layout(g->n, *mdp, nullptr, w);
   if (g->wsm_bytes < total) {
     if (g->wsm) hipFree(g->wsm);
     g->wsm = nullptr;
@@ -405,22 +451,7 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
     CG_HIP(hipMalloc(&g->wsm, total));
     g->wsm_bytes = total;
   }
-  char *p = (char *)g->wsm;
-  w->vis = (uint64_t *)p; p += wb;
-  w->fw[0] = (uint64_t *)p; p += wb;
-  w->fw[1] = (uint64_t *)p; p += wb;
-  w->fb = (uint32_t *)p; p += fbb;
-  w->q[0] = (int32_t *)p; p += qb;
-  w->q[1] = (int32_t *)p; p += qb;
-  w->ctrl = (MsCtrl *)p; p += al(sizeof(MsCtrl));
-  w->hist = (MsHist *)p; p += al(sizeof(MsHist));
-  w->lanes = (MsLane *)p; p += al(sizeof(MsLane));
-  w->mlane = (unsigned long long *)p; p += al(sizeof(unsigned long long) * 64);
-  w->cand = (uint32_t *)p; p += al(sizeof(uint32_t) * 64 * (nbig + 1));
-  w->bgot = (uint64_t *)p; p += al(sizeof(uint64_t) * (nbig + 1));
-  w->segq[0] = (int32_t *)p; p += al(sizeof(int32_t) * (md->pin.nseg + 1));
-  w->segq[1] = (int32_t *)p; p += al(sizeof(int32_t) * (md->pout.nseg + 1));
-  w->segneed = (uint64_t *)p;
+  ws_layout(g->n, *mdp, (char *)g->wsm, w);
   return CGBFS_OK;
 }
 
@@ -629,6 +660,24 @@ __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
   return r;
 }
 
+// The lane output table staged in LDS at kernel entry: the per-bit loops
+// read pointers and level values with broadcast LDS reads.
+struct LaneOutLds {
+  int32_t *par[64];
+  int16_t *lev[64];
+  int16_t level[64];  // the level a vertex reached this round gets in lane l
+};
+
+// Every thread of the block must call it (it ends with a block barrier).
+__device__ __forceinline__ void stage_out(const LaneOut *out, int d, LaneOutLds &s) {
+  if (threadIdx.x < 64) {
+    s.par[threadIdx.x] = out->par[threadIdx.x];
+    s.lev[threadIdx.x] = out->lev[threadIdx.x];
+    s.level[threadIdx.x] = (int16_t)(d + 1 - out->base[threadIdx.x]);
+  }
+  __syncthreads();
+}
+
 // Whether the direction policy needs the per-lane tallies.
 __host__ __device__ inline bool lane_policy(int mode) {
   return mode == CGBFS_DIR_TAU || mode == CGBFS_DIR_BEAMER_LANES;
@@ -742,13 +791,14 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
   const uint64_t pm = R.pm;
   const bool use_fb = R.use_fb;
   __shared__ unsigned long long red[kMsWaves];
+  __shared__ LaneOutLds so;
+  stage_out(A.out, d, so);
   __shared__ unsigned long long tl[192];
   tally_lds_init(tl);
   LaneTally tally;
   const bool sym = A.symmetric != 0;
   const uint64_t *fr = A.fw[d & 1];
   uint64_t *nx = A.fw[(d + 1) & 1];
-  const int16_t lev = (int16_t)(d + 1);
   // per-thread counts stay far below 2^32 (a thread owns a few rows per
   // chunk, each shorter than kSeg); block sums are 64-bit
   uint32_t scanned = 0, rows = 0, pairs = 0, mf = 0, nf = 0;
@@ -897,9 +947,8 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
         if (hb[k]) {
-          int64_t o = (int64_t)lb[k] * A.n + v;
-          A.parents[o] = pv[k];
-          A.levels[o] = lev;
+          so.par[lb[k]][v] = pv[k];
+          so.lev[lb[k]][v] = so.level[lb[k]];
         }
       }
     }
@@ -948,6 +997,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int tally_on) 
   if (!R.pm) return;
   const int d = R.d;
   __shared__ unsigned long long red[kMsWaves];
+  __shared__ LaneOutLds so;
+  stage_out(A.out, d, so);
   __shared__ unsigned long long tl[192];
   tally_lds_init(tl);
   LaneTally tally;
@@ -969,9 +1020,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int tally_on) 
         while (lanes) {
           int l = __ffsll((unsigned long long)lanes) - 1;
           lanes &= lanes - 1;
-          int64_t o = (int64_t)l * A.n + v;
-          A.parents[o] = (int32_t)A.cand[b * 64 + l];
-          A.levels[o] = (int16_t)(d + 1);
+          so.par[l][v] = (int32_t)A.cand[b * 64 + l];
+          so.lev[l][v] = so.level[l];
           A.cand[b * 64 + l] = 0xFFFFFFFFu;
         }
         A.bgot[b] = 0;
@@ -1104,6 +1154,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
   const int d = R.d;
   const uint64_t qm = R.qm;
   __shared__ unsigned long long red[kMsWaves];
+  __shared__ LaneOutLds so;
+  stage_out(A.out, d, so);
   const uint64_t *fr = A.fw[d & 1];
   uint64_t *nx = A.fw[(d + 1) & 1];
   const int32_t *Q = A.q[d & 1];
@@ -1160,7 +1212,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
           while (nv) {
             int l = __ffsll((unsigned long long)nv) - 1;
             nv &= nv - 1;
-            atomicMin((uint32_t *)&A.parents[(int64_t)l * A.n + w], (uint32_t)u);
+            atomicMin((uint32_t *)&so.par[l][w], (uint32_t)u);
           }
         }
       }
@@ -1184,6 +1236,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
   const int d = R.d;
   const uint64_t qm = R.qm;
   __shared__ unsigned long long red[kMsWaves];
+  __shared__ LaneOutLds so;
+  stage_out(A.out, d, so);
   __shared__ unsigned long long tl[192];
   tally_lds_init(tl);
   LaneTally tally;
@@ -1208,9 +1262,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
       while (lanes) {
         int l = __ffsll((unsigned long long)lanes) - 1;
         lanes &= lanes - 1;
-        int64_t o = (int64_t)l * A.n + v;
-        A.parents[o] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
-        A.levels[o] = (int16_t)(d + 1);
+        so.par[l][v] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
+        so.lev[l][v] = so.level[l];
       }
       A.vis[sv] |= nw;
       od = A.out_ptr[v + 1] - A.out_ptr[v];
@@ -1366,22 +1419,42 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
 // second stream while the first rounds run.  Plane-major: wavefront
 // item k is lane l = k / nch and vertices [64c, 64c + 64), c = k % nch, so
 // every output row is written front to back.
-__global__ void __launch_bounds__(kMsBlock) ms_fill_noin(int32_t *parents, int16_t *levels,
-                                                         const uint64_t *noin, int64_t n,
-                                                         int nroots) {
+// A list of lanes, passed by value.
+struct LaneList {
+  int32_t count;
+  int8_t lane[64];
+};
+
+__global__ void __launch_bounds__(kMsBlock) ms_fill_noin(const LaneOut *out, const uint64_t *noin,
+                                                         int64_t n, LaneList ll) {
+  constexpr int kGroup = 8;  // chunks per work item: their bitmap words load together
+  __shared__ int32_t *sp[64];
+  __shared__ int16_t *sv[64];
+  if ((int)threadIdx.x < ll.count) {
+    sp[threadIdx.x] = out->par[ll.lane[threadIdx.x]];
+    sv[threadIdx.x] = out->lev[ll.lane[threadIdx.x]];
+  }
+  __syncthreads();
   const int lane = lane_id();
   const int64_t nch = (n + kWave - 1) / kWave;
-  const int64_t total = nch * nroots;
+  const int64_t ngrp = (nch + kGroup - 1) / kGroup;
+  const int64_t total = ngrp * ll.count;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
   for (int64_t k = wave; k < total; k += nwaves) {
-    const int64_t l = k / nch, c = k - l * nch;
-    const uint64_t w = noin[c];
-    if (!w) continue;
-    const int64_t v = c * kWave + lane;  // a set bit implies v < n
-    if ((w >> lane) & 1ull) {
-      parents[l * n + v] = -1;
-      levels[l * n + v] = -1;
+    const int64_t j = k / ngrp, c0 = (k - j * ngrp) * kGroup;
+    uint64_t w[kGroup];
+#pragma unroll
+    for (int i = 0; i < kGroup; i++) w[i] = c0 + i < nch ? noin[c0 + i] : 0ull;
+    int32_t *pp = sp[j];
+    int16_t *lp = sv[j];
+#pragma unroll
+    for (int i = 0; i < kGroup; i++) {
+      if ((w[i] >> lane) & 1ull) {  // a set bit implies v < n
+        const int64_t v = (c0 + i) * kWave + lane;
+        pp[v] = -1;
+        lp[v] = -1;
+      }
     }
   }
 }
@@ -1390,9 +1463,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_fill_noin(int32_t *parents, int16
 __global__ void ms_seed_outputs(MsArgs A, const int64_t *sv, const int32_t *sl, int64_t nseeds) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nseeds) return;
-  const int64_t o = (int64_t)sl[k] * A.n + sv[k];
-  A.parents[o] = (int32_t)sv[k];
-  A.levels[o] = 0;
+  A.out->par[sl[k]][sv[k]] = (int32_t)sv[k];
+  A.out->lev[sl[k]][sv[k]] = 0;
 }
 
 // Final pass: -1 into every (lane, vertex) pair no lane word reached.  A
@@ -1412,9 +1484,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
       const int l = __ffsll((unsigned long long)bits) - 1;
       bits &= bits - 1;
       if ((miss >> l) & 1ull) {
-        const int64_t o = (int64_t)l * A.n + v;
-        A.parents[o] = -1;
-        A.levels[o] = -1;
+        A.out->par[l][v] = -1;
+        A.out->lev[l][v] = -1;
       }
     }
   }
@@ -1450,6 +1521,150 @@ __global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const 
   if (mine) atomicAdd(&acc[lane], mine);
   __syncthreads();
   if (threadIdx.x < (unsigned)nroots && acc[threadIdx.x]) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// Serving engine kernels: a long-lived pass whose lanes take queries as they
+// come (continuous batching).  Lanes join and leave at round boundaries.
+// ---------------------------------------------------------------------------
+
+// The queries seeded in one engine step, passed by value: lane lane[k]
+// starts at vertex root[k].
+struct SeedSet {
+  int32_t count;
+  int8_t lane[64];
+  int64_t root[64];
+};
+
+// Idle engine: no lane in flight, round counter at 0.  One wavefront.
+__global__ void eng_init(MsArgs A) {
+  MsCtrl *c = A.ctrl;
+  MsLane *L = A.lanes;
+  const int l = threadIdx.x;
+  if (l < 64) {
+    st_ctrl(&L->cnt[l], 0ull);
+    st_ctrl(&L->out[l], 0ull);
+    st_ctrl(&L->in[l], 0ull);
+    st_ctrl(&L->mu[l], 0ll);
+    st_ctrl(&L->prev[l], 0ll);
+    st_ctrl(&L->dir[l], 0);
+  }
+  if (l == 0) {
+    st_ctrl(&c->qtail[0], 0ull);
+    st_ctrl(&c->qtail[1], 0ull);
+    st_ctrl(&c->nf, 0ull);
+    st_ctrl(&c->scanned, 0ull);
+    st_ctrl(&c->rows, 0ull);
+    st_ctrl(&c->pairs, 0ull);
+    st_ctrl(&c->pnf, 0ull);
+    st_ctrl(&c->pscanned, 0ull);
+    st_ctrl(&c->prows, 0ull);
+    st_ctrl(&c->ppairs, 0ull);
+    st_ctrl(&c->mf, 0ull);
+    st_ctrl(&c->live, 0ull);
+    st_ctrl(&c->segtail[0], 0ull);
+    st_ctrl(&c->segtail[1], 0ull);
+    st_ctrl(&c->pmask, 0ull);
+    st_ctrl(&c->qmask, 0ull);
+    st_ctrl(&c->prev_nf, 0ll);
+    st_ctrl(&c->rounds, 0ll);
+    st_ctrl(&c->gdir, 0);
+    st_ctrl(&c->listed, 1);
+    st_ctrl(&c->use_fb, 0);
+    st_ctrl(&c->done, 1);
+    st_ctrl(&c->status, 0);
+  }
+}
+
+// Finishes the queries of lanes R: -1 into every vertex they did not reach
+// (vertices without in-edges were filled when the query was seeded), their
+// GTEPS m into mlane[l], and their bits cleared from the lane words so the
+// lanes can take new queries.  Wavefront layout as in ms_finalize (64
+// consecutive vertices, one coalesced store per lane bit) and ms_count (m
+// through an LDS transpose).
+__global__ void __launch_bounds__(kMsBlock) eng_retire(MsArgs A, uint64_t R, const int32_t *weight,
+                                                       unsigned long long *mlane) {
+  __shared__ LaneOutLds so;
+  stage_out(A.out, 0, so);
+  __shared__ unsigned long long acc[64];
+  __shared__ uint64_t sb[kMsWaves][kWave];
+  __shared__ uint32_t sw[kMsWaves][kWave];
+  if (threadIdx.x < 64) acc[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  unsigned long long mine = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < A.n; v0 += stride) {
+    const int64_t v = v0 + lane;
+    const bool inr = v < A.n;
+    const bool noin = inr && ((A.noin[v0 >> 6] >> lane) & 1ull);
+    const int32_t sv = inr ? A.slot_of[v] : 0;
+    const uint64_t w = inr ? A.vis[sv] : 0ull;
+    const uint64_t miss = (inr && !noin) ? R & ~w : 0ull;
+    uint64_t bits = wave_or64(miss);
+    while (bits) {
+      const int l = __ffsll((unsigned long long)bits) - 1;
+      bits &= bits - 1;
+      if ((miss >> l) & 1ull) {
+        so.par[l][v] = -1;
+        so.lev[l][v] = -1;
+      }
+    }
+    const uint64_t hit = w & R;
+    if (hit) A.vis[sv] = w & ~R;
+    if (!__any(hit != 0)) continue;
+    sb[wv][lane] = hit;
+    sw[wv][lane] = hit ? (uint32_t)weight[v] : 0u;
+    wave_sync();
+    for (int r = 0; r < kWave; r++)
+      if ((sb[wv][r] >> lane) & 1ull) mine += sw[wv][r];
+    wave_sync();
+  }
+  if (mine) atomicAdd(&acc[lane], mine);
+  __syncthreads();
+  if (threadIdx.x < 64 && acc[threadIdx.x]) atomicAdd(&mlane[threadIdx.x], acc[threadIdx.x]);
+}
+
+// Seeds the queries of S at the engine's current round: the lane words, the
+// lanes' direction state and level base, the roots' own outputs (after the
+// in-edge-free fill of their rows), and the control block.  The new lanes
+// join this round in their starting direction; the round rebuilds its
+// frontier list from the lane words, since the roots are not on it.  One
+// wavefront.
+__global__ void eng_seed(MsArgs A, LaneOut *out, SeedSet S, MsParams P) {
+  MsCtrl *c = A.ctrl;
+  MsLane *L = A.lanes;
+  const int t = threadIdx.x;
+  const int d = (int)ld_ctrl(&c->rounds);
+  uint64_t mine = 0;
+  int dir = 0;
+  if (t < S.count) {
+    const int l = S.lane[t];
+    const int64_t r = S.root[t];
+    const int32_t sr = A.slot_of[r];
+    mine = 1ull << l;
+    atomicOr((unsigned long long *)&A.vis[sr], (unsigned long long)mine);
+    atomicOr((unsigned long long *)&A.fw[d & 1][sr], (unsigned long long)mine);
+    out->base[l] = d;
+    out->par[l][r] = (int32_t)r;
+    out->lev[l][r] = 0;
+    dir = P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0;
+    st_ctrl(&L->cnt[l], 0ull);
+    st_ctrl(&L->out[l], 0ull);
+    st_ctrl(&L->in[l], 0ull);
+    st_ctrl(&L->mu[l], (long long)P.nnz - (long long)(A.in_ptr[r + 1] - A.in_ptr[r]));
+    st_ctrl(&L->prev[l], 1ll);
+    st_ctrl(&L->dir[l], dir);
+  }
+  const uint64_t added = wave_or64(mine);
+  const uint64_t pull = wave_or64(dir ? mine : 0ull);
+  if (t == 0 && added) {
+    st_ctrl(&c->pmask, (unsigned long long)(ld_ctrl(&c->pmask) | pull));
+    st_ctrl(&c->qmask, (unsigned long long)(ld_ctrl(&c->qmask) | (added & ~pull)));
+    st_ctrl(&c->listed, 0);
+    st_ctrl(&c->qtail[d & 1], 0ull);
+    st_ctrl(&c->done, 0);
+  }
 }
 
 __global__ void ms_zero(unsigned long long *p, int k) {
@@ -1503,8 +1718,13 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   A.out_idx = g->out_idx;
   A.in_ptr = g->in_ptr;
   A.in_idx = g->in_idx;
-  A.parents = dpar;
-  A.levels = dlev;
+  A.out = ws.out;
+  LaneOut lo;
+  std::memset(&lo, 0, sizeof(lo));
+  for (int l = 0; l < nroots; l++) {
+    lo.par[l] = dpar + (size_t)l * g->n;
+    lo.lev[l] = dlev + (size_t)l * g->n;
+  }
   A.vis = ws.vis;
   A.fw[0] = ws.fw[0];
   A.fw[1] = ws.fw[1];
@@ -1546,6 +1766,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   };
   hipEvent_t e0, e1;
   hipError_t e = hipMemcpyAsync(dsv, sv, sizeof(int64_t) * nseeds, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ws.out, &lo, sizeof(lo), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(dsl, sl, sizeof(int32_t) * nseeds, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = new_event(&e0);
   if (e == hipSuccess) e = new_event(&e1);
@@ -1557,7 +1778,10 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   if (e == hipSuccess) e = new_event(&side_done);
   if (e == hipSuccess) e = hipStreamWaitEvent(g->side, e0, 0);
   if (e == hipSuccess) {
-    ms_fill_noin<<<blocks, kMsBlock, 0, g->side>>>(dpar, dlev, md->noin, g->n, nroots);
+    LaneList all;
+    all.count = nroots;
+    for (int l = 0; l < nroots; l++) all.lane[l] = (int8_t)l;
+    ms_fill_noin<<<blocks, kMsBlock, 0, g->side>>>(ws.out, md->noin, g->n, all);
     side_used = true;
     e = hipGetLastError();
   }
@@ -1794,4 +2018,245 @@ extern "C" int cgbfs_msbfs_seeded(cgbfs_graph *h, const int64_t *verts, const in
     if (lanes[k] < 0 || lanes[k] >= nlanes) return set_error(CGBFS_EINVAL, "seed lane out of range");
   }
   return ms_run(h, verts, lanes, nseeds, nlanes, parents, levels, on_device, opt, stats, m_lanes);
+}
+
+// ---------------------------------------------------------------------------
+// Serving engine (continuous batching): host side
+// ---------------------------------------------------------------------------
+
+struct cgbfs_engine {
+  cgbfs_graph *h = nullptr;
+  MsGraphData *md = nullptr;
+  void *mem = nullptr;       // the engine's own multi-source workspace
+  MsWorkspace ws;
+  MsArgs A;
+  MsParams P;
+  int blocks = 0;
+  int tally = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  struct Pinned {            // read by the device copies of each step
+    MsCtrl hc;
+    unsigned long long m[64];
+    LaneOut out;
+  } *pin = nullptr;
+  uint64_t busy = 0;         // lanes holding a query (pending, running or retiring)
+  uint64_t pending = 0;      // submitted, seeded by the next step
+  uint64_t retiring = 0;     // finished, finalised by the next step
+  uint64_t retired = 0;      // finalised by the step in flight
+  int64_t root[64];
+  int64_t base[64];          // round each lane's query was seeded in
+  int32_t depth[64];         // rounds each lane's query ran
+  int64_t rounds_seen = 0;   // the device round counter at the last poll
+  bool inflight = false;
+};
+
+extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgbfs_engine **out) {
+  if (!h || !out) return set_error(CGBFS_EINVAL, "graph and out must not be null");
+  DeviceGraph *g = &h->g;
+  cgbfs_options o;
+  fill_options(opt, g->n, &o);
+  if (o.direction < 0 || o.direction > CGBFS_DIR_BEAMER_LANES)
+    return set_error(CGBFS_EINVAL, "unknown direction mode");
+  // lanes run at different depths: the union-count rule becomes the per-lane one
+  if (o.direction == CGBFS_DIR_BEAMER) o.direction = CGBFS_DIR_BEAMER_LANES;
+  CG_HIP(hipSetDevice(g->device));
+  cgbfs_engine *E = new cgbfs_engine();
+  E->h = h;
+  auto fail = [&](int rc) {
+    cgbfs_engine_destroy(E);
+    return rc;
+  };
+  int rc = ensure_msdata(g, &E->md);
+  if (rc) return fail(rc);
+  const size_t total = ws_layout(g->n, E->md, nullptr, &E->ws);
+  hipError_t e = hipMalloc(&E->mem, total);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&E->pin, sizeof(cgbfs_engine::Pinned), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&E->ev, hipEventDisableTiming);
+  if (e != hipSuccess) return fail(hip_fail(e, "engine create", __FILE__, __LINE__));
+  ws_layout(g->n, E->md, (char *)E->mem, &E->ws);
+  std::memset(E->pin, 0, sizeof(cgbfs_engine::Pinned));
+  MsArgs &A = E->A;
+  std::memset(&A, 0, sizeof(A));
+  A.out_ptr = g->out_ptr;
+  A.out_idx = g->out_idx;
+  A.in_ptr = g->in_ptr;
+  A.in_idx = g->in_idx;
+  A.out = E->ws.out;
+  A.vis = E->ws.vis;
+  A.fw[0] = E->ws.fw[0];
+  A.fw[1] = E->ws.fw[1];
+  A.fb = E->ws.fb;
+  A.q[0] = E->ws.q[0];
+  A.q[1] = E->ws.q[1];
+  A.ctrl = E->ws.ctrl;
+  A.hist = E->ws.hist;
+  A.lanes = E->ws.lanes;
+  A.pin = E->md->pin;
+  A.pout = E->md->pout;
+  A.slot_of = E->md->slot_of;
+  A.in_slot = E->md->in_slot;
+  A.out_slot = E->md->out_slot;
+  A.cand = E->ws.cand;
+  A.bgot = E->ws.bgot;
+  A.segq[0] = E->ws.segq[0];
+  A.segq[1] = E->ws.segq[1];
+  A.segneed = E->ws.segneed;
+  A.noin = E->md->noin;
+  A.host_done = nullptr;
+  A.n = g->n;
+  A.nroots = 64;
+  A.act = ~0ull;
+  A.symmetric = g->in_ptr == g->out_ptr ? 1 : 0;
+  MsParams &P = E->P;
+  std::memset(&P, 0, sizeof(P));
+  P.mode = o.direction;
+  P.alpha = o.alpha;
+  P.beta = o.beta;
+  P.tau = (long long)o.tau;
+  P.max_levels = 0x7FFFFFFFFFFFFFFFll;  // queries end on their own; levels are per query
+  P.nnz = (long long)g->nnz;
+  E->tally = lane_policy(P.mode) ? 1 : 0;
+  E->blocks = grid_blocks(g->device, o.grid_mult);
+  ms_init_fill<<<E->blocks, kMsBlock, 0, E->st>>>(A);
+  eng_init<<<1, 64, 0, E->st>>>(A);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(E->st);
+  if (e != hipSuccess) return fail(hip_fail(e, "engine init", __FILE__, __LINE__));
+  *out = E;
+  return CGBFS_OK;
+}
+
+extern "C" int cgbfs_engine_submit(cgbfs_engine *E, int32_t lane, int64_t root, int32_t *parents,
+                                   int16_t *levels) {
+  if (!E) return set_error(CGBFS_EINVAL, "engine is null");
+  if (lane < 0 || lane > 63) return set_error(CGBFS_EINVAL, "lane must be in [0, 63]");
+  if ((E->busy >> lane) & 1ull) return set_error(CGBFS_EINVAL, "lane is busy");
+  if (root < 0 || root >= E->h->g.n) return set_error(CGBFS_EINVAL, "root out of range");
+  if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
+  if (E->inflight) return set_error(CGBFS_EINVAL, "poll the step in flight first");
+  E->busy |= 1ull << lane;
+  E->pending |= 1ull << lane;
+  E->root[lane] = root;
+  E->pin->out.par[lane] = parents;
+  E->pin->out.lev[lane] = levels;
+  return CGBFS_OK;
+}
+
+extern "C" int cgbfs_engine_step(cgbfs_engine *E) {
+  if (!E) return set_error(CGBFS_EINVAL, "engine is null");
+  if (E->inflight) return set_error(CGBFS_EINVAL, "poll the step in flight first");
+  DeviceGraph *g = &E->h->g;
+  CG_HIP(hipSetDevice(g->device));
+  MsArgs &A = E->A;
+  hipStream_t st = E->st;
+  const int blocks = E->blocks;
+  ms_zero<<<1, 64, 0, st>>>(E->ws.mlane, 64);
+  // 1. finish the lanes whose search ended in the last step (the device table
+  //    still holds their output rows)
+  if (E->retiring) {
+    eng_retire<<<blocks, kMsBlock, 0, st>>>(A, E->retiring, g->m_weight, E->ws.mlane);
+  }
+  E->retired = E->retiring;
+  E->retiring = 0;
+  // 2. seed the queued queries: output rows, in-edge-free entries, lane state.
+  //    The whole table is uploaded, so the host copy carries every lane's
+  //    level base too (the device round counter equals rounds_seen here).
+  if (E->pending) {
+    SeedSet S;
+    LaneList ll;
+    S.count = 0;
+    for (int l = 0; l < 64; l++)
+      if ((E->pending >> l) & 1ull) {
+        S.lane[S.count] = (int8_t)l;
+        S.root[S.count] = E->root[l];
+        ll.lane[S.count] = (int8_t)l;
+        S.count++;
+        E->base[l] = E->rounds_seen;
+        E->pin->out.base[l] = (int32_t)E->rounds_seen;
+      }
+    ll.count = S.count;
+    hipError_t e = hipMemcpyAsync(E->ws.out, &E->pin->out, sizeof(LaneOut), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "engine step", __FILE__, __LINE__);
+    ms_fill_noin<<<blocks, kMsBlock, 0, st>>>(E->ws.out, E->md->noin, g->n, ll);
+    eng_seed<<<1, 64, 0, st>>>(A, E->ws.out, S, E->P);
+    E->pending = 0;
+  }
+  // 3. one round for every lane in flight (each kernel gates itself)
+  auto sblocks = [&](int64_t nb) {
+    int64_t k = (nb + kMsBlock - 1) / kMsBlock;
+    return (unsigned)(k < blocks ? k : blocks);
+  };
+  MsGraphData *md = E->md;
+  ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
+  if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
+  if (E->tally) ms_pull<true><<<blocks, kMsBlock, 0, st>>>(A);
+  else ms_pull<false><<<blocks, kMsBlock, 0, st>>>(A);
+  if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, E->tally);
+  ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
+  if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
+  ms_push<<<blocks, kMsBlock, 0, st>>>(A, 0);
+  ms_prep<<<blocks, kMsBlock, 0, st>>>(A, E->tally);
+  ms_push<<<blocks, kMsBlock, 0, st>>>(A, 1);
+  ms_clear<<<blocks, kMsBlock, 0, st>>>(A);
+  ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A);
+  ms_decide<<<1, 64, 0, st>>>(A, E->P);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&E->pin->hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(E->pin->m, E->ws.mlane, sizeof(E->pin->m), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipEventRecord(E->ev, st);
+  if (e != hipSuccess) return hip_fail(e, "engine step", __FILE__, __LINE__);
+  E->inflight = true;
+  return CGBFS_OK;
+}
+
+extern "C" int cgbfs_engine_poll(cgbfs_engine *E, int32_t wait, int32_t *ready, uint64_t *completed,
+                                 uint64_t *busy, int64_t *m_lanes, int32_t *depth_lanes) {
+  if (!E || !ready) return set_error(CGBFS_EINVAL, "engine and ready must not be null");
+  *ready = 0;
+  if (completed) *completed = 0;
+  if (busy) *busy = E->busy;
+  if (!E->inflight) {
+    *ready = 1;
+    return CGBFS_OK;
+  }
+  hipError_t e = wait ? hipEventSynchronize(E->ev) : hipEventQuery(E->ev);
+  if (e == hipErrorNotReady) return CGBFS_OK;
+  if (e != hipSuccess) return hip_fail(e, "engine poll", __FILE__, __LINE__);
+  E->inflight = false;
+  *ready = 1;
+  const MsCtrl &hc = E->pin->hc;
+  if (hc.status) return set_error(hc.status, "engine round failed");
+  // lanes retired by this step: outputs final, m counted
+  const uint64_t done = E->retired;
+  for (int l = 0; l < 64; l++)
+    if ((done >> l) & 1ull) {
+      if (m_lanes) m_lanes[l] = (int64_t)E->pin->m[l];
+      if (depth_lanes) depth_lanes[l] = E->depth[l];
+    }
+  E->busy &= ~done;
+  E->retired = 0;
+  // lanes whose frontier emptied this round finish in the next step
+  E->rounds_seen = hc.rounds;
+  const uint64_t running = E->busy & ~E->pending & ~E->retiring;
+  const uint64_t live = hc.done ? 0ull : (uint64_t)(hc.pmask | hc.qmask);
+  const uint64_t ended = running & ~live;
+  for (int l = 0; l < 64; l++)
+    if ((ended >> l) & 1ull) E->depth[l] = (int32_t)(hc.rounds - E->base[l]);
+  E->retiring |= ended;
+  if (completed) *completed = done;
+  if (busy) *busy = E->busy;
+  return CGBFS_OK;
+}
+
+extern "C" int cgbfs_engine_destroy(cgbfs_engine *E) {
+  if (!E) return CGBFS_OK;
+  if (E->st) (void)hipStreamSynchronize(E->st);
+  if (E->ev) (void)hipEventDestroy(E->ev);
+  if (E->st) (void)hipStreamDestroy(E->st);
+  if (E->pin) (void)hipHostFree(E->pin);
+  if (E->mem) (void)hipFree(E->mem);
+  delete E;
+  return CGBFS_OK;
 }

@@ -278,6 +278,66 @@ class DeviceGraph:
     return MSBFSResult(verts, par, lev, m, sb.result())
 
 
+class ServingEngine:
+  """A long-lived 64-lane multi-source search that takes single-root
+  queries as lanes free up (cgbfs_engine_*): submit() queues a query on a
+  free lane, step() runs one round for every lane in flight (asynchronous),
+  poll() reports the queries completed by that step."""
+
+  LANES = 64
+
+  def __init__(self, graph, direction="beamer_lanes", **kw):
+    self.graph = graph
+    self.n = graph.n
+    h = ctypes.c_void_p()
+    opt = _options(direction, **kw)
+    _lib.check(_lib.load().cgbfs_engine_create(graph._h, ctypes.byref(opt), ctypes.byref(h)))
+    self._e = h
+    self._m = np.zeros(64, np.int64)
+    self._depth = np.zeros(64, np.int32)
+
+  def submit(self, lane, root, parents, levels):
+    """parents: int32 CUDA tensor of n elements, levels: int16, both on the
+    graph's device; they receive this query's outputs."""
+    _check_tensor(parents, self.n, "int32")
+    _check_tensor(levels, self.n, "int16")
+    _lib.check(_lib.load().cgbfs_engine_submit(self._e, int(lane), int(root),
+                                               ctypes.c_void_p(parents.data_ptr()),
+                                               ctypes.c_void_p(levels.data_ptr())))
+
+  def step(self):
+    _lib.check(_lib.load().cgbfs_engine_step(self._e))
+
+  def poll(self, wait=False):
+    """(ready, completed lanes, busy lanes, m per lane, rounds per lane);
+    the last two are valid for the completed lanes."""
+    ready = ctypes.c_int32()
+    done = ctypes.c_uint64()
+    busy = ctypes.c_uint64()
+    _lib.check(_lib.load().cgbfs_engine_poll(self._e, 1 if wait else 0, ctypes.byref(ready),
+                                             ctypes.byref(done), ctypes.byref(busy),
+                                             _ptr(self._m), _ptr(self._depth)))
+    lanes = [l for l in range(64) if (done.value >> l) & 1]
+    return bool(ready.value), lanes, busy.value, self._m.copy(), self._depth.copy()
+
+  def close(self):
+    if self._e:
+      _lib.load().cgbfs_engine_destroy(self._e)
+      self._e = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:  # pylint: disable=broad-except
+      pass
+
+  def __enter__(self):
+    return self
+
+  def __exit__(self, *exc):
+    self.close()
+
+
 def _check_tensor(t, numel, dtype):
   import torch  # plumbing only: device memory for outputs that stay in HBM
   want = {"int32": torch.int32, "int16": torch.int16}[dtype]

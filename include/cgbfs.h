@@ -177,6 +177,46 @@ CGBFS_API int cgbfs_msbfs_seeded(cgbfs_graph *g, const int64_t *verts, const int
                        int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats,
                        int64_t *m_lanes);
 
+/* --- serving engine: continuous batching --------------------------------- */
+
+typedef struct cgbfs_engine cgbfs_engine;
+
+/* A long-lived 64-lane multi-source search over one graph that serves a
+ * stream of single-root queries.  A query takes a free lane at the next
+ * round boundary, runs interleaved with the queries already in flight (one
+ * adjacency pass per round serves them all), and leaves when its frontier
+ * empties.  Each query writes its own device buffers, parents int32[n] and
+ * levels int16[n], with results identical to cgbfs_bfs from the same root:
+ * the lowest-id rule does not depend on the other lanes.  Directions are
+ * chosen per lane (CGBFS_DIR_BEAMER acts as CGBFS_DIR_BEAMER_LANES), since
+ * lanes are at different depths.  The engine owns its workspace and HIP
+ * stream; do not run other searches on the graph while it is live.  No
+ * counterpart in the reference, which runs one search per launch
+ * (run_bfs.py:288-504); its callers are serving loops. */
+CGBFS_API int cgbfs_engine_create(cgbfs_graph *g, const cgbfs_options *opt, cgbfs_engine **out);
+
+/* Queues a query on a free lane (0..63) for the next step.  CGBFS_EINVAL if
+ * the lane is busy or a step is in flight (poll it first). */
+CGBFS_API int cgbfs_engine_submit(cgbfs_engine *e, int32_t lane, int64_t root, int32_t *parents,
+                        int16_t *levels);
+
+/* Enqueues one step on the engine's stream and returns at once: the lanes
+ * whose search ended in the previous step are finalised (their outputs
+ * become complete), queued queries are seeded, and one BFS round runs for
+ * every lane in flight. */
+CGBFS_API int cgbfs_engine_step(cgbfs_engine *e);
+
+/* Reports on the step in flight; wait != 0 blocks until it completes.
+ * *ready = 0: still running.  Once ready: *completed = lanes whose queries
+ * completed in that step (outputs final, lane free again), with their GTEPS
+ * m in m_lanes[l] and their round count (max level + 1) in depth_lanes[l]
+ * (either array may be NULL, else 64 entries); *busy = lanes still holding
+ * a query. */
+CGBFS_API int cgbfs_engine_poll(cgbfs_engine *e, int32_t wait, int32_t *ready, uint64_t *completed,
+                      uint64_t *busy, int64_t *m_lanes, int32_t *depth_lanes);
+
+CGBFS_API int cgbfs_engine_destroy(cgbfs_engine *e);
+
 /* Blocks until all work queued on the graph's stream has finished. */
 CGBFS_API int cgbfs_sync(cgbfs_graph *g);
 

@@ -1,0 +1,80 @@
+"""Serving engine (continuous batching) and the serving loop.
+
+GPU: queries that join a running engine at different rounds, take lanes
+freed by finished queries and run interleaved with each other produce the
+same parents and levels as the same query run alone (cgbfs_bfs) and as the
+oracle, with the oracle's m and round count.  CPU: the arrival stream and
+the latency statistics of the serving loop."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ceregraphs_amd import gen_rmat, serve
+from ceregraphs_amd.engine import DeviceGraph, ServingEngine
+
+
+def test_poisson_stream_is_reproducible_and_shaped():
+  nonisolated = [np.arange(1, 50), np.arange(0, 200, 2)]
+  a = serve.poisson_stream(1000.0, 5000, nonisolated, seed=20)
+  b = serve.poisson_stream(1000.0, 5000, nonisolated, seed=20)
+  np.testing.assert_array_equal(a.t, b.t)
+  np.testing.assert_array_equal(a.graph, b.graph)
+  np.testing.assert_array_equal(a.root, b.root)
+  assert np.all(np.diff(a.t) >= 0)
+  assert abs(np.mean(np.diff(a.t)) - 1e-3) < 1e-4          # rate 1k/s
+  for g, cand in enumerate(nonisolated):
+    assert np.isin(a.root[a.graph == g], cand).all()
+  assert 0.4 < np.mean(a.graph == 0) < 0.6                  # uniform over graphs
+
+
+def test_latency_summary():
+  s = serve.latency_summary(np.array([1.0, 2.0, 3.0, 4.0, 100.0]) * 1e-3)
+  assert s["p50_ms"] == 3.0 and s["max_ms"] == 100.0
+  assert 4.0 < s["p99_ms"] <= 100.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scale,ef,seed,directed,direction", [
+    (13, 16, 21, False, "beamer_lanes"), (12, 8, 22, True, "beamer_lanes"),
+    (12, 16, 23, False, "topdown"), (11, 16, 24, False, "bottomup"), (12, 8, 25, True, "tau")])
+def test_engine_results_equal_alone_and_in_mix(scale, ef, seed, directed, direction):
+  import torch
+  abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+  g = O.Graph(n, ip, ii, op, oi, not directed)
+  rng = np.random.default_rng(seed)
+  nq = 150  # more queries than lanes: lanes are reused
+  roots = rng.integers(0, n, size=nq)
+  roots[:3] = [int(np.argmax(np.diff(op))), roots[5], roots[5]]  # a hub, a repeated root
+  par = [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(nq)]
+  lev = [torch.empty(n, dtype=torch.int16, device="cuda") for _ in range(nq)]
+  with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg, \
+       ServingEngine(dg, direction=direction) as eng:
+    free, lane_of, nxt, done = list(range(64)), {}, 0, {}
+    step = 0
+    while len(done) < nq:
+      # staggered arrivals: up to 7 new queries per step while lanes are free
+      for _ in range(7):
+        if nxt < nq and free:
+          lane = free.pop(0)
+          eng.submit(lane, int(roots[nxt]), par[nxt], lev[nxt])
+          lane_of[lane] = nxt
+          nxt += 1
+      eng.step()
+      ready, completed, busy, m, depth = eng.poll(wait=True)
+      assert ready
+      for lane in completed:
+        q = lane_of.pop(lane)
+        done[q] = (int(m[lane]), int(depth[lane]))
+        free.append(lane)
+      step += 1
+      assert step < 100000
+    for q in range(nq):
+      lev0, par0, _, rounds = O.bfs(g, int(roots[q]))
+      np.testing.assert_array_equal(lev[q].cpu().numpy(), lev0.astype(np.int16), err_msg=f"query {q}")
+      np.testing.assert_array_equal(par[q].cpu().numpy(), par0, err_msg=f"query {q}")
+      assert done[q] == (O.m_edges(g, lev0), rounds), f"query {q}"
+    for q in range(0, nq, 15):  # the same query alone, single-root kernel
+      alone = dg.bfs(int(roots[q]))
+      np.testing.assert_array_equal(lev[q].cpu().numpy(), alone.levels.astype(np.int16))
+      np.testing.assert_array_equal(par[q].cpu().numpy(), alone.parents)
