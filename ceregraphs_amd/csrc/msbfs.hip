@@ -2035,6 +2035,7 @@ struct cgbfs_engine {
   int tally = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
+  hipGraphExec_t round = nullptr;  // one round + the copies back, captured once
   struct Pinned {            // read by the device copies of each step
     MsCtrl hc;
     unsigned long long m[64];
@@ -2050,6 +2051,31 @@ struct cgbfs_engine {
   int64_t rounds_seen = 0;   // the device round counter at the last poll
   bool inflight = false;
 };
+
+// One round for every lane in flight: the gated kernel sequence.
+static void eng_round(cgbfs_engine *E) {
+  const MsArgs &A = E->A;
+  hipStream_t st = E->st;
+  const int blocks = E->blocks;
+  MsGraphData *md = E->md;
+  auto sblocks = [&](int64_t nb) {
+    int64_t k = (nb + kMsBlock - 1) / kMsBlock;
+    return (unsigned)(k < blocks ? k : blocks);
+  };
+  ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
+  if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
+  if (E->tally) ms_pull<true><<<blocks, kMsBlock, 0, st>>>(A);
+  else ms_pull<false><<<blocks, kMsBlock, 0, st>>>(A);
+  if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, E->tally);
+  ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
+  if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
+  ms_push<<<blocks, kMsBlock, 0, st>>>(A, 0);
+  ms_prep<<<blocks, kMsBlock, 0, st>>>(A, E->tally);
+  ms_push<<<blocks, kMsBlock, 0, st>>>(A, 1);
+  ms_clear<<<blocks, kMsBlock, 0, st>>>(A);
+  ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A);
+  ms_decide<<<1, 64, 0, st>>>(A, E->P);
+}
 
 extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgbfs_engine **out) {
   if (!h || !out) return set_error(CGBFS_EINVAL, "graph and out must not be null");
@@ -2124,6 +2150,22 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(E->st);
   if (e != hipSuccess) return fail(hip_fail(e, "engine init", __FILE__, __LINE__));
+  // The round is the same launch sequence every step (each kernel gates
+  // itself on the control block): capture it once, with the copies of the
+  // control block and the m counters back to pinned memory.
+  hipGraph_t graph = nullptr;
+  e = hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    eng_round(E);
+    e = hipMemcpyAsync(&E->pin->hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, E->st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(E->pin->m, E->ws.mlane, sizeof(E->pin->m), hipMemcpyDeviceToHost, E->st);
+    hipError_t ec = hipStreamEndCapture(E->st, &graph);
+    if (e == hipSuccess) e = ec;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&E->round, graph, nullptr, nullptr, 0);
+  if (graph) hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(hip_fail(e, "engine round capture", __FILE__, __LINE__));
   *out = E;
   return CGBFS_OK;
 }
@@ -2183,28 +2225,9 @@ extern "C" int cgbfs_engine_step(cgbfs_engine *E) {
     eng_seed<<<1, 64, 0, st>>>(A, E->ws.out, S, E->P);
     E->pending = 0;
   }
-  // 3. one round for every lane in flight (each kernel gates itself)
-  auto sblocks = [&](int64_t nb) {
-    int64_t k = (nb + kMsBlock - 1) / kMsBlock;
-    return (unsigned)(k < blocks ? k : blocks);
-  };
-  MsGraphData *md = E->md;
-  ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
-  if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
-  if (E->tally) ms_pull<true><<<blocks, kMsBlock, 0, st>>>(A);
-  else ms_pull<false><<<blocks, kMsBlock, 0, st>>>(A);
-  if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, E->tally);
-  ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
-  if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
-  ms_push<<<blocks, kMsBlock, 0, st>>>(A, 0);
-  ms_prep<<<blocks, kMsBlock, 0, st>>>(A, E->tally);
-  ms_push<<<blocks, kMsBlock, 0, st>>>(A, 1);
-  ms_clear<<<blocks, kMsBlock, 0, st>>>(A);
-  ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A);
-  ms_decide<<<1, 64, 0, st>>>(A, E->P);
+  // 3. one round for every lane in flight, and the copies back (captured)
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(&E->pin->hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(E->pin->m, E->ws.mlane, sizeof(E->pin->m), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipGraphLaunch(E->round, st);
   if (e == hipSuccess) e = hipEventRecord(E->ev, st);
   if (e != hipSuccess) return hip_fail(e, "engine step", __FILE__, __LINE__);
   E->inflight = true;
@@ -2253,6 +2276,7 @@ extern "C" int cgbfs_engine_poll(cgbfs_engine *E, int32_t wait, int32_t *ready, 
 extern "C" int cgbfs_engine_destroy(cgbfs_engine *E) {
   if (!E) return CGBFS_OK;
   if (E->st) (void)hipStreamSynchronize(E->st);
+  if (E->round) (void)hipGraphExecDestroy(E->round);
   if (E->ev) (void)hipEventDestroy(E->ev);
   if (E->st) (void)hipStreamDestroy(E->st);
   if (E->pin) (void)hipHostFree(E->pin);
