@@ -8,6 +8,11 @@ l is exactly the single-root search of graph g from its root l: 8 graphs x
 32 roots = 256 searches in one 32-lane pass.  The reference instead runs one
 search per launch on one graph at a time (bfs/scripts/run_graph500.py:246-375).
 
+Each graph is first relabelled by the shared-permutation balancer
+(balance.py; util/analyze --shared-perm, or --symmetric for undirected
+graphs) on a P x P grid, as the reference prepares its inputs, and roots are
+drawn in the balanced ids.
+
 The pass runs in graph-replay mode (the round kernels are captured once and
 replayed, every level decided on the device).  `autotune` picks the
 direction policy, alpha/beta and the blocks per CU for a batch by timing
@@ -39,6 +44,7 @@ UNEVEN_SPECS = [(12, 8, 10), (14, 32, 11), (15, 16, 12), (16, 8, 13), (17, 16, 1
                 (18, 32, 15), (19, 8, 16), (20, 16, 17)]
 UNEVEN_ROOT_SEED = 18
 UNEVEN_ROOTS_PER_GRAPH = 32
+UNEVEN_BALANCE_GRID = 4  # P of the P x P balancing grid (0: generator ids)
 
 DEFAULT_CONFIG = {"direction": "beamer", "alpha": 14.0, "beta": 24.0, "grid_mult": 8}
 
@@ -67,6 +73,20 @@ def union_csr(parts):
   return n, in_ptr, in_idx, out_ptr, out_idx, offs
 
 
+def balance_part(part, grid, symmetric):
+  """The graph relabelled by the balancer's shared permutation q, computed
+  on M = CSR(in-view) (M[dst][src], the Matrix Market orientation) with
+  --symmetric for an undirected graph, else --shared-perm.  Returns the
+  balanced part over grid * ceil(n / grid) vertices (the padding vertices
+  have no edges) and the balancer's result (q = .operm)."""
+  from . import balance
+  n, ip, ii, op, oi = part
+  res = balance.balance_csr(n, ip, ii, grid, symmetric=symmetric)
+  bip, bii = balance.relabel_csr(ip, ii, res.operm)
+  bop, boi = balance.relabel_csr(op, oi, res.operm)
+  return (res.n_padded, bip, bii, bop, boi), res
+
+
 def pick_roots(out_degrees, per_graph, seed):
   """per_graph roots for every graph, among vertices with out-degree >= 1,
   from one numpy Generator(seed) drawn graph after graph in order (the
@@ -90,6 +110,7 @@ class UnevenBatch:
 
   def __init__(self, parts, symmetric, device=0, graphs=None):
     self.graphs = graphs or []  # the parts as separate device graphs, if kept
+    self.balance = []           # the balancer's report per graph, if balanced
     self.ns = [int(p[0]) for p in parts]
     n, ip, ii, op, oi, offs = union_csr(parts)
     self.n, self.offsets, self.symmetric = n, offs, symmetric
@@ -104,17 +125,31 @@ class UnevenBatch:
       self.dg = DeviceGraph.from_csr(n, op, oi, ip, ii, symmetric=False, device=device)
 
   @classmethod
-  def from_rmat(cls, specs, abcd=DIRECTED_ABCD, directed=True, device=0, keep_parts=False):
-    parts, graphs = [], []
+  def from_rmat(cls, specs, abcd=DIRECTED_ABCD, directed=True, device=0, keep_parts=False,
+                balance_grid=UNEVEN_BALANCE_GRID):
+    """Generates the graphs on the GPU, balances each on a balance_grid x
+    balance_grid grid (0: keep the generator's ids) and packs the union.
+    keep_parts also keeps every graph as a device graph of its own."""
+    parts, graphs, reports = [], [], []
     for scale, ef, seed in specs:
       g = DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed, device=device)
       ip, ii, op, oi = g.download()
-      parts.append((g.n, ip, ii, op, oi))
+      part = (g.n, ip, ii, op, oi)
+      if balance_grid:
+        g.close()
+        part, res = balance_part(part, balance_grid, symmetric=not directed)
+        reports.append(res.report)
+        if keep_parts:
+          g = DeviceGraph.from_csr(part[0], part[3], part[4], part[1], part[2],
+                                   symmetric=not directed, device=device)
+      parts.append(part)
       if keep_parts:
         graphs.append(g)
-      else:
+      elif not balance_grid:
         g.close()
-    return cls(parts, symmetric=not directed, device=device, graphs=graphs)
+    batch = cls(parts, symmetric=not directed, device=device, graphs=graphs)
+    batch.balance = reports
+    return batch
 
   def close(self):
     self.dg.close()
@@ -196,11 +231,13 @@ def main(argv=None):
   p.add_argument("--check", type=int, default=8,
                  help="searches checked against single-root GPU searches (0: none)")
   p.add_argument("--no-tune", action="store_true")
+  p.add_argument("--balance-grid", type=int, default=UNEVEN_BALANCE_GRID,
+                 help="P of the P x P --shared-perm balancing (0: generator ids)")
   args = p.parse_args(argv)
   import torch
 
   t0 = time.perf_counter()
-  batch = UnevenBatch.from_rmat(UNEVEN_SPECS, keep_parts=True)
+  batch = UnevenBatch.from_rmat(UNEVEN_SPECS, keep_parts=True, balance_grid=args.balance_grid)
   roots = pick_roots(batch.graph_out_degrees(), UNEVEN_ROOTS_PER_GRAPH, UNEVEN_ROOT_SEED)
   lanes = UNEVEN_ROOTS_PER_GRAPH
   print(f"[batch] {len(UNEVEN_SPECS)} graphs, union n={batch.n} nnz={batch.dg.nnz}, "
@@ -252,7 +289,8 @@ def main(argv=None):
   msum = float(m.sum())
   report = {"workload": "uneven-8-directed-rmat", "searches": int(nsearch),
             "union_vertices": batch.n, "union_entries": batch.dg.nnz,
-            "rounds": res.rounds}
+            "rounds": res.rounds, "balance_grid": args.balance_grid,
+            "balance_lb": [round(b["lb"], 4) for b in batch.balance]}
   for name, (cfg, wall, ev) in lines.items():
     hm = timing.harmonic_mean(m.ravel() / (wall / 1e3 / nsearch) / 1e9)
     report[name] = {"config": cfg, "ms_per_pass_wall": round(wall, 4),

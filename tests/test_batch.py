@@ -56,6 +56,24 @@ def test_search_inside_union_equals_search_on_the_graph():
       assert (ulev[:lo] == -1).all() and (ulev[hi:] == -1).all()
 
 
+def test_balanced_part_is_a_relabelling():
+  """balance_part gives the same searches under q: levels[q[v]] from q[r]
+  equal levels[v] from r; padding vertices stay unreached."""
+  part = _parts([(10, 8, 3)])[0]
+  bpart, res = B.balance_part(part, 3, symmetric=False)  # 1024 -> 3 * 342 = 1026 vertices
+  q = res.operm
+  assert bpart[0] == 1026 and res.report["grid"] == 3
+  g0 = O.Graph(*part, False)
+  g1 = O.Graph(*bpart, False)
+  for r in B.pick_roots([np.diff(part[3])], 4, 18)[0]:
+    lev0, par0, hist0, _ = O.bfs(g0, int(r))
+    lev1, par1, hist1, _ = O.bfs(g1, int(q[r]))
+    np.testing.assert_array_equal(lev1[q[:part[0]]], lev0)
+    assert (lev1[np.setdiff1d(np.arange(1026), q[:part[0]])] == -1).all()
+    np.testing.assert_array_equal(hist1, hist0)
+    assert O.validate(g1, int(q[r]), par1, lev1) == []
+
+
 def test_pick_roots_rule():
   degs = [np.array([0, 2, 1, 0, 5]), np.array([1, 1, 0])]
   roots = B.pick_roots(degs, 2, 18)
@@ -67,10 +85,13 @@ def test_pick_roots_rule():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("directed,specs,per_graph", [
-    (True, SMALL_DIRECTED, 32), (False, [(9, 16, 21), (7, 8, 22), (10, 16, 23)], 20)])
-def test_batch_searches_equal_oracle(directed, specs, per_graph):
+@pytest.mark.parametrize("directed,specs,per_graph,grid", [
+    (True, SMALL_DIRECTED, 32, 0), (False, [(9, 16, 21), (7, 8, 22), (10, 16, 23)], 20, 0),
+    (True, SMALL_DIRECTED, 16, 3)])
+def test_batch_searches_equal_oracle(directed, specs, per_graph, grid):
   parts = _parts(specs, directed)
+  if grid:
+    parts = [B.balance_part(p, grid, symmetric=not directed)[0] for p in parts]
   with B.UnevenBatch(parts, symmetric=not directed) as batch:
     roots = B.pick_roots(batch.graph_out_degrees(), per_graph, 18)
     for capture, cfg in [(True, {}), (False, {}), (True, {"direction": "beamer_lanes"}),
