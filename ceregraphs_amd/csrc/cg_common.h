@@ -101,6 +101,14 @@ __device__ __forceinline__ void st_ctrl(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Search outputs (parents, levels) are written once and not read again by
+// the search: a non-temporal store keeps them from evicting the lane words
+// that the gathers of later rounds reuse.
+template <class T>
+__device__ __forceinline__ void st_once(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -947,8 +947,8 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
         if (hb[k]) {
-          so.par[lb[k]][v] = pv[k];
-          so.lev[lb[k]][v] = so.level[lb[k]];
+          st_once(&so.par[lb[k]][v], pv[k]);
+          st_once(&so.lev[lb[k]][v], so.level[lb[k]]);
         }
       }
     }
@@ -1020,8 +1020,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int tally_on) 
         while (lanes) {
           int l = __ffsll((unsigned long long)lanes) - 1;
           lanes &= lanes - 1;
-          so.par[l][v] = (int32_t)A.cand[b * 64 + l];
-          so.lev[l][v] = so.level[l];
+          st_once(&so.par[l][v], (int32_t)A.cand[b * 64 + l]);
+          st_once(&so.lev[l][v], so.level[l]);
           A.cand[b * 64 + l] = 0xFFFFFFFFu;
         }
         A.bgot[b] = 0;
@@ -1452,8 +1452,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_fill_noin(const LaneOut *out, con
     for (int i = 0; i < kGroup; i++) {
       if ((w[i] >> lane) & 1ull) {  // a set bit implies v < n
         const int64_t v = (c0 + i) * kWave + lane;
-        pp[v] = -1;
-        lp[v] = -1;
+        st_once(&pp[v], (int32_t)-1);
+        st_once(&lp[v], (int16_t)-1);
       }
     }
   }
@@ -1484,8 +1484,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
       const int l = __ffsll((unsigned long long)bits) - 1;
       bits &= bits - 1;
       if ((miss >> l) & 1ull) {
-        A.out->par[l][v] = -1;
-        A.out->lev[l][v] = -1;
+        st_once(&A.out->par[l][v], (int32_t)-1);
+        st_once(&A.out->lev[l][v], (int16_t)-1);
       }
     }
   }
@@ -1606,8 +1606,8 @@ __global__ void __launch_bounds__(kMsBlock) eng_retire(MsArgs A, uint64_t R, con
       const int l = __ffsll((unsigned long long)bits) - 1;
       bits &= bits - 1;
       if ((miss >> l) & 1ull) {
-        so.par[l][v] = -1;
-        so.lev[l][v] = -1;
+        st_once(&so.par[l][v], (int32_t)-1);
+        st_once(&so.lev[l][v], (int16_t)-1);
       }
     }
     const uint64_t hit = w & R;
