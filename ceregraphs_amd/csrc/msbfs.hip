@@ -55,8 +55,13 @@ constexpr int kMsHist = 4096;
 constexpr int64_t kSeg = 256;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
-constexpr int kFlush = 8;
-constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)           // lane bits per batch of the pull's output flush
+constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
+constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
+// The pull hands out its 64-vertex chunks dynamically from kParts cursors,
+// one per contiguous part of the vertex range, kept a cache line apart.
+constexpr int kParts = 8;
+constexpr int kCursorStride = 16;
+constexpr int32_t kGrabSparse = 8;  // chunks per pull grab when the pull is known to be light
 // First-hit offsets within a wavefront-tier row (< kSeg) fit in a byte.
 using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
@@ -85,6 +90,7 @@ struct MsCtrl {
   int use_fb;                  // that round's pull reads the frontier bits
   int done;
   int status;
+  int grab;                    // chunks per grab of that round's pull
 };
 
 // Every round kernel reads its round from the control block (the previous
@@ -95,6 +101,7 @@ struct RoundState {
   int d;
   uint64_t pm, qm;
   bool listed, use_fb;
+  int grab;
 };
 
 // Per-lane direction state.  cnt/out/in are this round's tallies of the
@@ -170,11 +177,14 @@ struct MsArgs {
   SegList pin, pout;
   int32_t *segq[2];
   uint64_t *segneed;  // [pin.nseg] lanes needed by each active pull segment
+  unsigned long long *cursor;  // [kParts * kCursorStride] pull chunk cursors, zero between rounds
   const uint64_t *noin;  // vertices without in-edges, one bit each
   int *host_done;        // pinned host flag raised with done (graph replay), or null
   // Vertex-state slot of each vertex (hub-first order) and the adjacency
-  // arrays re-expressed in slots, entry for entry; the lane words vis/fw
-  // are indexed by slot, everything else by vertex id.
+  // arrays re-expressed in slots, entry for entry.  The frontier words fw,
+  // which the pull gathers through in_slot, are indexed by slot, so the
+  // most-gathered words share a few MB; vis and everything else is indexed
+  // by vertex id, so per-vertex passes read it in order.
   const int32_t *slot_of;
   const int32_t *in_slot;
   const int32_t *out_slot;
@@ -219,6 +229,7 @@ struct MsWorkspace {
   uint64_t *bgot;
   int32_t *segq[2];
   uint64_t *segneed;
+  unsigned long long *cursor;
 };
 
 // ---------------------------------------------------------------------------
@@ -397,7 +408,7 @@ static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspac
       al(sizeof(LaneOut)), al(sizeof(unsigned long long) * 64),
       al(sizeof(uint32_t) * 64 * (nbig + 1)), al(sizeof(uint64_t) * (nbig + 1)),
       al(sizeof(int32_t) * (md->pin.nseg + 1)), al(sizeof(int32_t) * (md->pout.nseg + 1)),
-      al(sizeof(uint64_t) * (md->pin.nseg + 1))};
+      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(unsigned long long) * kParts * kCursorStride)};
   size_t total = 0;
   for (size_t x : sizes) total += x;
   if (!base) return total;
@@ -420,6 +431,7 @@ static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspac
   w->segq[0] = (int32_t *)take();
   w->segq[1] = (int32_t *)take();
   w->segneed = (uint64_t *)take();
+  w->cursor = (unsigned long long *)take();
   return total;
 }
 
@@ -650,6 +662,57 @@ __global__ void __launch_bounds__(kMsBlock) ms_init_fill(MsArgs A) {
   for (int64_t i = tid; i < A.pin.nbig; i += stride) A.bgot[i] = 0;
 }
 
+// Zeroes the pull's chunk cursors for the next round (one thread).
+__device__ __forceinline__ void reset_cursors(unsigned long long *cur) {
+  for (int p = 0; p < kParts; ++p) st_ctrl(&cur[p * kCursorStride], 0ull);
+}
+
+// Hands out the pull's 64-vertex chunks.  Part p holds chunks
+// [nch*p/kParts, nch*(p+1)/kParts).  A wavefront draws from its home part
+// first, then from the other parts in turn, so the parts finish together
+// whatever their costs.  A grab is one chunk in rounds with real scanning (a
+// dense round has only a few dozen chunks per wavefront, each costing
+// hundreds of microseconds, and needs single chunks, taken only when the
+// previous one is done, to balance) and kGrabSparse chunks in a round known
+// to be light, where one atomic per chunk would cost more than the chunk;
+// there the next grab is issued before the current chunks are worked on, so
+// its latency hides behind theirs.  ms_decide picks the size.  Blocks reach
+// the XCDs round-robin, so with home = blockIdx.x % kParts each XCD mostly
+// walks one contiguous stretch of vertices.  Wavefront-uniform.
+struct ChunkFeed {
+  unsigned long long *cur;
+  int32_t nch;
+  int home;
+  int32_t want;    // chunks per grab
+  int tried;       // parts found drained so far
+  bool pending;    // a grab is in flight
+  int32_t issued;  // lane 0: the cursor value it returned
+
+  __device__ int part() const { return (home + tried) % kParts; }
+  __device__ void issue() {
+    if (lane_id() == 0)
+      issued = (int32_t)atomicAdd(&cur[part() * kCursorStride], (unsigned long long)want);
+    pending = true;
+  }
+  // The next chunks, [b, e); false once every part is drained.
+  __device__ bool next(int32_t &b, int32_t &e) {
+    for (;;) {
+      if (!pending) issue();
+      pending = false;
+      const int p = part();
+      const int64_t lo = (int64_t)nch * p / kParts, hi = (int64_t)nch * (p + 1) / kParts;
+      const int64_t k = lo + __shfl(issued, 0);
+      if (k < hi) {
+        b = (int32_t)k;
+        e = (int32_t)(k + want < hi ? k + want : hi);
+        if (want > 1) issue();
+        return true;
+      }
+      if (++tried == kParts) return false;
+    }
+  }
+};
+
 __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
   RoundState r;
   r.d = (int)ld_ctrl(&c->rounds);
@@ -657,6 +720,7 @@ __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
   r.qm = (uint64_t)ld_ctrl(&c->qmask);
   r.listed = ld_ctrl(&c->listed) != 0;
   r.use_fb = ld_ctrl(&c->use_fb) != 0;
+  r.grab = ld_ctrl(&c->grab);
   return r;
 }
 
@@ -705,7 +769,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     const int64_t r = sv[k];
     const int32_t sr = A.slot_of[r];
     const unsigned long long bit = 1ull << sl[k];
-    atomicOr((unsigned long long *)&A.vis[sr], bit);
+    atomicOr((unsigned long long *)&A.vis[r], bit);
     if (atomicOr((unsigned long long *)&A.fw[0][sr], bit) == 0) {
       A.q[0][atomicAdd(&s_nq, 1ull)] = (int32_t)r;
       atomicAdd(&s_deg, (unsigned long long)(A.out_ptr[r + 1] - A.out_ptr[r]));
@@ -728,6 +792,8 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->rounds, 0ll);
     st_ctrl(&c->done, 0);
     st_ctrl(&c->status, 0);
+    st_ctrl(&c->grab, 1);
+    reset_cursors(A.cursor);
   }
   bool live = false;
   int dir = 0;
@@ -819,7 +885,13 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
   // gives each (row, lane) its first -- lowest-id -- hit.  Row state (got)
   // lives in the lane that owns the row.
   const int32_t nch = (int32_t)((A.n + kWave - 1) / kWave);
-  for (int32_t ch = wave; ch < nch; ch += nwaves) {
+  // Chunks come from the cursors: their costs differ by orders of magnitude
+  // (row lengths, how soon rows are satisfied), which a static split of the
+  // chunks over the wavefronts leaves unbalanced.
+  ChunkFeed feed{A.cursor, nch, (int)(blockIdx.x % kParts), R.grab, 0, false, 0};
+  int32_t cb = 0, ce = 0;
+  while (cb < ce || feed.next(cb, ce)) {
+    const int32_t ch = cb++;
     const int32_t v = ch * kWave + lane;  // vertex ids are int32
     // branch-free loads (a lane past n reads vertex 0): the per-vertex
     // loads go out together, then the lane word through the slot
@@ -828,7 +900,7 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
     const bool big = A.pin.is_big[vv] != 0;
     const int32_t sv_me = A.slot_of[vv];
     const int64_t rb0 = A.in_ptr[vv], re0 = A.in_ptr[vv + 1];
-    const uint64_t vw = A.vis[sv_me];
+    const uint64_t vw = A.vis[vv];
     const uint64_t need = (inr && !big) ? pm & ~vw : 0ull;
     const int64_t rb = rb0;
     const int32_t len = need ? (int32_t)(re0 - rb0) : 0;  // < kSeg for non-big rows
@@ -955,7 +1027,7 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
     int64_t od = 0;
     if (got) {
       nx[sv_me] = got;
-      A.vis[sv_me] = vw | got;
+      A.vis[v] = vw | got;
       od = sym ? (int64_t)len : A.out_ptr[v + 1] - A.out_ptr[v];
       mf += (uint32_t)od;
       live |= got;
@@ -1025,9 +1097,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int tally_on) 
           A.cand[b * 64 + l] = 0xFFFFFFFFu;
         }
         A.bgot[b] = 0;
-        const int32_t sv = A.slot_of[v];
-        nx[sv] = got;
-        A.vis[sv] |= got;
+        nx[A.slot_of[v]] = got;
+        A.vis[v] |= got;
         id = A.in_ptr[v + 1] - A.in_ptr[v];
         od = A.symmetric ? id : A.out_ptr[v + 1] - A.out_ptr[v];
         mf += (unsigned long long)od;
@@ -1089,8 +1160,9 @@ __global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int which) 
     int32_t first = 0, cnt = 0;
     uint64_t need = 0;
     if (b < L.nbig) {
-      const int32_t sv = A.slot_of[L.big_v[b]];
-      need = mask & ~A.vis[sv];
+      const int32_t bv = L.big_v[b];
+      const int32_t sv = A.slot_of[bv];
+      need = mask & ~A.vis[bv];
       const bool on = which ? (fr[sv] & mask) != 0 : need != 0;
       if (on) {
         first = L.seg_first[b];
@@ -1202,7 +1274,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
         w = A.out_idx[e];
         const int32_t sw = A.out_slot[e];
         if (phase == 0) {
-          uint64_t nv = f & ~A.vis[sw];
+          uint64_t nv = f & ~A.vis[w];
           if (nv) {
             uint64_t old = atomicOr((unsigned long long *)&nx[sw], (unsigned long long)nv);
             first = (old & qm) == 0;
@@ -1265,7 +1337,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
         so.par[l][v] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
         so.lev[l][v] = so.level[l];
       }
-      A.vis[sv] |= nw;
+      A.vis[v] |= nw;
       od = A.out_ptr[v + 1] - A.out_ptr[v];
       id = A.symmetric ? od : A.in_ptr[v + 1] - A.in_ptr[v];
       if ((all & ~qm) == 0) {
@@ -1363,11 +1435,12 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   }
   if (l != 0) return;
   const unsigned long long cur_pm = ld_ctrl(&c->pmask), cur_qm = ld_ctrl(&c->qmask);
+  const long long pscanned = (long long)ld_ctrl(&c->pscanned);
   if (d < kMsHist) {
     MsHist *H = A.hist;
     H->nf[d] = nf;
     H->pnf[d] = pnf;
-    H->pscanned[d] = (long long)ld_ctrl(&c->pscanned);
+    H->pscanned[d] = pscanned;
     H->prows[d] = (long long)ld_ctrl(&c->prows);
     H->ppairs[d] = (long long)ld_ctrl(&c->ppairs);
     H->scanned[d] = H->pscanned[d] + (long long)ld_ctrl(&c->scanned);
@@ -1388,6 +1461,10 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   st_ctrl(&c->live, 0ull);
   st_ctrl(&c->segtail[0], 0ull);
   st_ctrl(&c->segtail[1], 0ull);
+  reset_cursors(A.cursor);
+  // A pull rescans every row it opens from the start, so the next pull reads
+  // at most what this one read: below one entry per vertex it is light.
+  st_ctrl(&c->grab, cur_pm && pscanned < (long long)A.n ? kGrabSparse : 1);
   // A pull part lists none of its new vertices: the next round's frontier is
   // listed only if this round did not pull, and a push part after a pull
   // rebuilds the list from an empty tail.
@@ -1478,7 +1555,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
     const int64_t v = v0 + lane;
     // vertices without in-edges are written by ms_fill_noin
     const bool skip = v >= A.n || ((A.noin[v0 >> 6] >> lane) & 1ull);
-    const uint64_t miss = skip ? 0ull : A.act & ~A.vis[A.slot_of[v]];
+    const uint64_t miss = skip ? 0ull : A.act & ~A.vis[v];
     uint64_t bits = wave_or64(miss);
     while (bits) {
       const int l = __ffsll((unsigned long long)bits) - 1;
@@ -1495,8 +1572,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
 // set.  A wavefront stages its 64 vertices' (word, weight) pairs in LDS and
 // lane l then walks them for lane bit l: a transpose through LDS with
 // broadcast reads, no cross-lane reductions.
-__global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const int32_t *slot_of,
-                                                     const int32_t *w, int64_t n, int nroots,
+__global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const int32_t *w, int64_t n,
+                                                     int nroots,
                                                      unsigned long long *out) {
   __shared__ unsigned long long acc[64];
   __shared__ uint64_t sb[kMsWaves][kWave];
@@ -1509,7 +1586,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_count(const uint64_t *vis, const 
   for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < n; v0 += stride) {
     const int64_t v = v0 + lane;
     const uint32_t wt = v < n ? (uint32_t)w[v] : 0u;
-    const uint64_t b = wt ? vis[slot_of[v]] : 0ull;
+    const uint64_t b = wt ? vis[v] : 0ull;
     if (!__any(b != 0)) continue;
     sb[wv][lane] = b;
     sw[wv][lane] = wt;
@@ -1573,6 +1650,8 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&c->use_fb, 0);
     st_ctrl(&c->done, 1);
     st_ctrl(&c->status, 0);
+    st_ctrl(&c->grab, 1);
+    reset_cursors(A.cursor);
   }
 }
 
@@ -1598,8 +1677,7 @@ __global__ void __launch_bounds__(kMsBlock) eng_retire(MsArgs A, uint64_t R, con
     const int64_t v = v0 + lane;
     const bool inr = v < A.n;
     const bool noin = inr && ((A.noin[v0 >> 6] >> lane) & 1ull);
-    const int32_t sv = inr ? A.slot_of[v] : 0;
-    const uint64_t w = inr ? A.vis[sv] : 0ull;
+    const uint64_t w = inr ? A.vis[v] : 0ull;
     const uint64_t miss = (inr && !noin) ? R & ~w : 0ull;
     uint64_t bits = wave_or64(miss);
     while (bits) {
@@ -1611,7 +1689,7 @@ __global__ void __launch_bounds__(kMsBlock) eng_retire(MsArgs A, uint64_t R, con
       }
     }
     const uint64_t hit = w & R;
-    if (hit) A.vis[sv] = w & ~R;
+    if (hit) A.vis[v] = w & ~R;
     if (!__any(hit != 0)) continue;
     sb[wv][lane] = hit;
     sw[wv][lane] = hit ? (uint32_t)weight[v] : 0u;
@@ -1643,7 +1721,7 @@ __global__ void eng_seed(MsArgs A, LaneOut *out, SeedSet S, MsParams P) {
     const int64_t r = S.root[t];
     const int32_t sr = A.slot_of[r];
     mine = 1ull << l;
-    atomicOr((unsigned long long *)&A.vis[sr], (unsigned long long)mine);
+    atomicOr((unsigned long long *)&A.vis[r], (unsigned long long)mine);
     atomicOr((unsigned long long *)&A.fw[d & 1][sr], (unsigned long long)mine);
     out->base[l] = d;
     out->par[l][r] = (int32_t)r;
@@ -1677,6 +1755,21 @@ using namespace cg;
 
 // The multi-source search behind both entry points: nseeds seeds (vertex
 // sv[k] starts lane sl[k]) over nroots lanes.
+// Blocks of a pull launch: at most as many as are resident on the device at
+// once.  The pull hands out its chunks dynamically, so blocks that could only
+// start after others finish would add a tail and nothing else.
+static int pull_grid(int device, int blocks, bool tally) {
+  int occ = 0, cus = 0;
+  const hipError_t e = tally
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms_pull<true>, kMsBlock, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms_pull<false>, kMsBlock, 0);
+  if (e != hipSuccess || occ <= 0 ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      cus <= 0)
+    return blocks;
+  return std::min(blocks, occ * cus);
+}
+
 static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t nseeds,
                   int32_t nroots, int32_t *parents, int16_t *levels, int32_t on_device,
                   const cgbfs_options *opt, cgbfs_stats *stats, int64_t *m_lanes) {
@@ -1744,6 +1837,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   A.segq[0] = ws.segq[0];
   A.segq[1] = ws.segq[1];
   A.segneed = ws.segneed;
+  A.cursor = ws.cursor;
   A.noin = md->noin;
   A.host_done = nullptr;
   A.n = g->n;
@@ -1752,6 +1846,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   A.symmetric = g->in_ptr == g->out_ptr ? 1 : 0;
 
   const int blocks = grid_blocks(g->device, o.grid_mult);
+  const int pblocks = pull_grid(g->device, blocks, lane_policy(o.direction));
   std::vector<hipEvent_t> evs;
   auto new_event = [&](hipEvent_t *e) -> hipError_t {
     hipError_t r = hipEventCreate(e);
@@ -1816,8 +1911,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
   };
   auto pull_kernels = [&]() {
-    if (tally) ms_pull<true><<<blocks, kMsBlock, 0, st>>>(A);
-    else ms_pull<false><<<blocks, kMsBlock, 0, st>>>(A);
+    if (tally) ms_pull<true><<<pblocks, kMsBlock, 0, st>>>(A);
+    else ms_pull<false><<<pblocks, kMsBlock, 0, st>>>(A);
     if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, tally);
   };
   auto push_part = [&](bool compact) {
@@ -1977,8 +2072,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   if (m_lanes) {
     unsigned long long m[64];
     ms_zero<<<1, 64, 0, st>>>(ws.mlane, 64);
-    ms_count<<<grid_blocks(g->device, 8), kMsBlock, 0, st>>>(ws.vis, md->slot_of, g->m_weight, g->n,
-                                                             nroots, ws.mlane);
+    ms_count<<<grid_blocks(g->device, 8), kMsBlock, 0, st>>>(ws.vis, g->m_weight, g->n, nroots,
+                                                             ws.mlane);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(m, ws.mlane, sizeof(m), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -2032,6 +2127,7 @@ struct cgbfs_engine {
   MsArgs A;
   MsParams P;
   int blocks = 0;
+  int pblocks = 0;  // pull launches
   int tally = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
@@ -2064,8 +2160,8 @@ static void eng_round(cgbfs_engine *E) {
   };
   ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
   if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
-  if (E->tally) ms_pull<true><<<blocks, kMsBlock, 0, st>>>(A);
-  else ms_pull<false><<<blocks, kMsBlock, 0, st>>>(A);
+  if (E->tally) ms_pull<true><<<E->pblocks, kMsBlock, 0, st>>>(A);
+  else ms_pull<false><<<E->pblocks, kMsBlock, 0, st>>>(A);
   if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, E->tally);
   ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
   if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
@@ -2129,6 +2225,7 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   A.segq[0] = E->ws.segq[0];
   A.segq[1] = E->ws.segq[1];
   A.segneed = E->ws.segneed;
+  A.cursor = E->ws.cursor;
   A.noin = E->md->noin;
   A.host_done = nullptr;
   A.n = g->n;
@@ -2145,6 +2242,7 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   P.nnz = (long long)g->nnz;
   E->tally = lane_policy(P.mode) ? 1 : 0;
   E->blocks = grid_blocks(g->device, o.grid_mult);
+  E->pblocks = pull_grid(g->device, E->blocks, E->tally != 0);
   ms_init_fill<<<E->blocks, kMsBlock, 0, E->st>>>(A);
   eng_init<<<1, 64, 0, E->st>>>(A);
   e = hipGetLastError();
