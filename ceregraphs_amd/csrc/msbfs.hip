@@ -57,6 +57,10 @@ constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
 constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
+// Blocks per CU of the side-stream output fill (ms_fill_noin).  One block
+// leaves room for the round kernels on every CU: a full grid of fill blocks
+// held the CUs and delayed the first round kernels by the whole fill.
+constexpr int kNoinFillPerCU = 1;
 // The pull hands out its 64-vertex chunks dynamically from kParts cursors,
 // one per contiguous part of the vertex range, kept a cache line apart.
 constexpr int kParts = 8;
@@ -470,24 +474,78 @@ static int ensure_wsm(DeviceGraph *g, MsWorkspace *w, MsGraphData **mdp) {
 // ---------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t shfl_up64(uint64_t x, int off) {
-  uint32_t lo = __shfl_up((uint32_t)x, off), hi = __shfl_up((uint32_t)(x >> 32), off);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
   uint32_t lo = __shfl((uint32_t)x, src), hi = __shfl((uint32_t)(x >> 32), src);
   return ((uint64_t)hi << 32) | lo;
 }
 
+// A 64-bit word moved across lanes by one DPP operation (VALU, no LDS
+// round trip): kCtrl is the GFX9 DPP control, rows outside kRowMask and
+// lanes whose source lies outside the pattern read 0.
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ uint64_t dpp64(uint64_t x) {
+  const uint32_t lo =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, kCtrl, kRowMask, 0xF, true);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), kCtrl, kRowMask, 0xF, true);
+  return ((uint64_t)hi << 32) | lo;
+}
+constexpr int kDppRowShr = 0x110;     // + k: lane j reads lane j-k of its 16-lane row
+constexpr int kDppWaveShr1 = 0x138;   // lane j reads lane j-1
+constexpr int kDppRowBcast15 = 0x142; // rows 1, 3 (row mask 0xA) read lane 15 of the row before
+constexpr int kDppRowBcast31 = 0x143; // rows 2, 3 (row mask 0xC) read lane 31
+
+// Segmented inclusive prefix-OR across the wavefront: lane j gets the OR of
+// x over lanes [head, j], where head <= j is the first lane of j's segment
+// (segments are runs of consecutive lanes).  Shifts within 16-lane rows,
+// then the row broadcasts carry each row's tail into the rows after it,
+// each step gated on the source lane lying in the same segment.  Needs
+// every lane of the wavefront active.
+__device__ __forceinline__ uint64_t seg_scan_or(uint64_t x, int head, int lane) {
+  uint64_t y = dpp64<kDppRowShr + 1>(x);
+  if (lane - 1 >= head) x |= y;
+  y = dpp64<kDppRowShr + 2>(x);
+  if (lane - 2 >= head) x |= y;
+  y = dpp64<kDppRowShr + 4>(x);
+  if (lane - 4 >= head) x |= y;
+  y = dpp64<kDppRowShr + 8>(x);
+  if (lane - 8 >= head) x |= y;
+  y = dpp64<kDppRowBcast15, 0xA>(x);
+  if (head < (lane & ~15)) x |= y;
+  y = dpp64<kDppRowBcast31, 0xC>(x);
+  if (head < 32) x |= y;
+  return x;
+}
+
+// The word of lane j-1 (0 for lane 0).
+__device__ __forceinline__ uint64_t lane_before(uint64_t x) { return dpp64<kDppWaveShr1>(x); }
+
 // Inclusive prefix-OR across the wavefront.
-__device__ __forceinline__ uint64_t wave_scan_or(uint64_t x) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    uint64_t y = shfl_up64(x, off);
-    if (lane >= off) x |= y;
-  }
+__device__ __forceinline__ uint64_t wave_scan_or(uint64_t x) { return seg_scan_or(x, 0, lane_id()); }
+
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ int32_t dpp32(int32_t x) {
+  return __builtin_amdgcn_update_dpp(0, x, kCtrl, kRowMask, 0xF, true);
+}
+
+// Inclusive prefix sum / prefix max of non-negative values across the
+// wavefront (0 is the identity of both on non-negative values).
+__device__ __forceinline__ int32_t wave_scan_add(int32_t x) {
+  x += dpp32<kDppRowShr + 1>(x);
+  x += dpp32<kDppRowShr + 2>(x);
+  x += dpp32<kDppRowShr + 4>(x);
+  x += dpp32<kDppRowShr + 8>(x);
+  x += dpp32<kDppRowBcast15, 0xA>(x);
+  x += dpp32<kDppRowBcast31, 0xC>(x);
+  return x;
+}
+__device__ __forceinline__ int32_t wave_scan_max(int32_t x) {
+  x = max(x, dpp32<kDppRowShr + 1>(x));
+  x = max(x, dpp32<kDppRowShr + 2>(x));
+  x = max(x, dpp32<kDppRowShr + 4>(x));
+  x = max(x, dpp32<kDppRowShr + 8>(x));
+  x = max(x, dpp32<kDppRowBcast15, 0xA>(x));
+  x = max(x, dpp32<kDppRowBcast31, 0xC>(x));
   return x;
 }
 
@@ -518,13 +576,12 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
   return v;
 }
 
+// OR of x over the wavefront, in every lane (every lane must be active).
 __device__ __forceinline__ uint64_t wave_or64(uint64_t x) {
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    uint32_t lo = __shfl_xor((uint32_t)x, off), hi = __shfl_xor((uint32_t)(x >> 32), off);
-    x |= ((uint64_t)hi << 32) | lo;
-  }
-  return x;
+  x = wave_scan_or(x);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, kWave - 1);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), kWave - 1);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint32_t clamp_deg(int64_t d) {
@@ -633,8 +690,7 @@ __device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t 
     const uint64_t h = ok[k] ? w[k] & need & ~got : 0ull;
     if (!__any(h != 0)) continue;  // no frontier in-neighbour in this step
     const uint64_t inc = wave_scan_or(h);
-    uint64_t before = shfl_up64(inc, 1);
-    if (lane == 0) before = 0;
+    const uint64_t before = lane_before(inc);
     uint64_t mine = h & ~before;
     while (mine) {
       int l = __ffsll((unsigned long long)mine) - 1;
@@ -877,6 +933,9 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
   // reads one conflict-free row of LDS and stores 64 consecutive vertices.
   __shared__ TileOff ptile[kMsWaves][kWave][kWave];
   TileOff (*pt)[kWave] = ptile[threadIdx.x / kWave];
+  // Row marks of the current pass, [window][position].
+  __shared__ int32_t rmap[kMsWaves][kUnroll][kWave];
+  int32_t (*rm)[kWave] = rmap[threadIdx.x / kWave];
   // Chunks of 64 consecutive vertices per wavefront.  The rows of the chunk
   // that still need lanes (big rows excluded) are laid end to end in a
   // virtual edge range [0, total); each step every lane takes one virtual
@@ -906,16 +965,14 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
     const int32_t len = need ? (int32_t)(re0 - rb0) : 0;  // < kSeg for non-big rows
     if (len) rows++;
     // exclusive prefix sum of row lengths -> virtual starts
-    int32_t incl = len;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      int32_t y = __shfl_up(incl, off);
-      if (lane >= off) incl += y;
-    }
+    const int32_t incl = wave_scan_add(len);
     const int32_t vstart = incl - len;
-    const int32_t total = __shfl(incl, kWave - 1);
+    const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
     uint64_t got = 0;
     const int32_t my_end = vstart + len;
+    // Row of the entries before the current window: the windows run in
+    // order, so it is the row of the previous window's last lane.
+    int rcarry = 0;
     // kUnroll steps per pass: their index loads, then their frontier-word
     // gathers, are all in flight together (no branches around the loads:
     // idle lanes read entry 0 and the word of slot 0, both cached); the
@@ -925,19 +982,28 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
       int32_t rsk[kUnroll], idx[kUnroll];
       bool ok[kUnroll];
       uint64_t nrk[kUnroll], raw[kUnroll];
+      // Row of each entry of the pass: each row owner marks where its row
+      // starts in the pass, and a prefix-max over the marks (carried across
+      // windows) gives every entry the last row starting at or before it.
+      // Rows of length 0 mark nothing, so the row found always has entries.
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) rm[k][lane] = 0;
+      wave_sync();
+      {
+        const int32_t p = vstart - base0;
+        if (len && p >= 0 && p < kWave * kUnroll) rm[p / kWave][p % kWave] = lane;
+      }
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) {
+        const int r = max(wave_scan_max(rm[k][lane]), rcarry);
+        rcarry = __builtin_amdgcn_readlane(r, kWave - 1);
+        rk[k] = r;
+      }
 #pragma unroll
       for (int k = 0; k < kUnroll; k++) {
         const int32_t ve = base0 + k * kWave + lane;
-        // row r: the largest lane with vstart <= ve (rows of length 0 share
-        // the next row's start, so r always has entries)
-        int r = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-          int cand = r + step;
-          int32_t sv = __shfl(vstart, cand);
-          if (sv <= ve) r = cand;
-        }
-        rk[k] = r;
+        const int r = rk[k];
         rsk[k] = __shfl(vstart, r);
         const int64_t rbase = __shfl(rb, r);
         nrk[k] = shfl64(need, r);
@@ -970,13 +1036,8 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
         uint64_t h = raw[k] & ~shfl64(got, r);
         // segmented inclusive prefix-OR; the segment of lane starts at head
         const int head = rs > base ? rs - base : 0;
-        uint64_t x = h;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          uint64_t y = shfl_up64(x, off);
-          if (lane - off >= head) x |= y;
-        }
-        uint64_t prev = shfl_up64(x, 1);
+        const uint64_t x = seg_scan_or(h, head, lane);
+        const uint64_t prev = lane_before(x);
         uint64_t mine = h & ~(lane > head ? prev : 0ull);
         const TileOff off = (TileOff)(base + lane - rs);
         while (mine) {
@@ -993,12 +1054,7 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
       if (!__any(len && (need & ~got))) break;  // every row of the chunk satisfied
     }
     // Flush: per lane bit, 64 consecutive vertices' parents and levels.
-    uint64_t bits = got;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      uint32_t lo = __shfl_xor((uint32_t)bits, off), hi = __shfl_xor((uint32_t)(bits >> 32), off);
-      bits |= ((uint64_t)hi << 32) | lo;
-    }
+    uint64_t bits = wave_or64(got);
     // kFlush bits at a time: their parent-id loads are in flight together
     // before any store is issued.
     while (bits) {
@@ -1876,7 +1932,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     LaneList all;
     all.count = nroots;
     for (int l = 0; l < nroots; l++) all.lane[l] = (int8_t)l;
-    ms_fill_noin<<<blocks, kMsBlock, 0, g->side>>>(ws.out, md->noin, g->n, all);
+    ms_fill_noin<<<grid_blocks(g->device, kNoinFillPerCU), kMsBlock, 0, g->side>>>(ws.out, md->noin,
+                                                                                 g->n, all);
     side_used = true;
     e = hipGetLastError();
   }
