@@ -101,12 +101,22 @@ __device__ __forceinline__ void st_ctrl(T *p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The global-memory view of a pointer.  Accesses through it compile to
+// global_* instructions.  Through a generic pointer that was itself read from
+// LDS or memory (the per-lane output tables) they compile to flat_* ones,
+// which also count against the LDS wait counter: every later wait for an LDS
+// read then waits for those stores or atomics to complete as well.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 // Search outputs (parents, levels) are written once and not read again by
 // the search: a non-temporal store keeps them from evicting the lane words
 // that the gathers of later rounds reuse.
 template <class T>
 __device__ __forceinline__ void st_once(T *p, T v) {
-  __builtin_nontemporal_store(v, p);
+  __builtin_nontemporal_store(v, gmem(p));
 }
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }

@@ -1340,7 +1340,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
           while (nv) {
             int l = __ffsll((unsigned long long)nv) - 1;
             nv &= nv - 1;
-            atomicMin((uint32_t *)&so.par[l][w], (uint32_t)u);
+            __hip_atomic_fetch_min(gmem((uint32_t *)&so.par[l][w]), (uint32_t)u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
           }
         }
       }
@@ -1390,8 +1391,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
       while (lanes) {
         int l = __ffsll((unsigned long long)lanes) - 1;
         lanes &= lanes - 1;
-        so.par[l][v] = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
-        so.lev[l][v] = so.level[l];
+        *gmem(&so.par[l][v]) = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
+        *gmem(&so.lev[l][v]) = so.level[l];
       }
       A.vis[v] |= nw;
       od = A.out_ptr[v + 1] - A.out_ptr[v];
@@ -1780,8 +1781,8 @@ __global__ void eng_seed(MsArgs A, LaneOut *out, SeedSet S, MsParams P) {
     atomicOr((unsigned long long *)&A.vis[r], (unsigned long long)mine);
     atomicOr((unsigned long long *)&A.fw[d & 1][sr], (unsigned long long)mine);
     out->base[l] = d;
-    out->par[l][r] = (int32_t)r;
-    out->lev[l][r] = 0;
+    *gmem(&out->par[l][r]) = (int32_t)r;
+    *gmem(&out->lev[l][r]) = (int16_t)0;
     dir = P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0;
     st_ctrl(&L->cnt[l], 0ull);
     st_ctrl(&L->out[l], 0ull);
