@@ -57,10 +57,6 @@ constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
 constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
-// Blocks per CU of the side-stream output fill (ms_fill_noin).  One block
-// leaves room for the round kernels on every CU: a full grid of fill blocks
-// held the CUs and delayed the first round kernels by the whole fill.
-constexpr int kNoinFillPerCU = 1;
 // The pull hands out its 64-vertex chunks dynamically from kParts cursors,
 // one per contiguous part of the vertex range, kept a cache line apart.
 constexpr int kParts = 8;
@@ -95,6 +91,7 @@ struct MsCtrl {
   int done;
   int status;
   int grab;                    // chunks per grab of that round's pull
+  int noin_left;               // outputs of vertices without in-edges not yet written
 };
 
 // Every round kernel reads its round from the control block (the previous
@@ -106,6 +103,7 @@ struct RoundState {
   uint64_t pm, qm;
   bool listed, use_fb;
   int grab;
+  bool noin;  // this round's pull also writes the outputs of vertices without in-edges
 };
 
 // Per-lane direction state.  cnt/out/in are this round's tallies of the
@@ -777,6 +775,7 @@ __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
   r.listed = ld_ctrl(&c->listed) != 0;
   r.use_fb = ld_ctrl(&c->use_fb) != 0;
   r.grab = ld_ctrl(&c->grab);
+  r.noin = ld_ctrl(&c->noin_left) != 0;
   return r;
 }
 
@@ -807,7 +806,7 @@ __host__ __device__ inline bool lane_policy(int mode) {
 // sl[k]) in the lane words and lists their distinct vertices; thread l sets
 // lane l's direction state the way the single-root search starts (round 0
 // pushes unless every round pulls, bool_pe.csl:714-719).  The seeds' own
-// outputs are written by ms_seed_outputs after the output fill.
+// outputs are written by ms_seed_outputs after the last round.
 __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, int64_t nseeds,
                               MsParams P) {
   if (blockIdx.x != 0) return;
@@ -849,6 +848,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->done, 0);
     st_ctrl(&c->status, 0);
     st_ctrl(&c->grab, 1);
+    st_ctrl(&c->noin_left, 1);
     reset_cursors(A.cursor);
   }
   bool live = false;
@@ -1053,8 +1053,13 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
       }
       if (!__any(len && (need & ~got))) break;  // every row of the chunk satisfied
     }
+    // Vertices without in-edges: no lane but a seed's own ever reaches them.
+    // The first pull round writes their -1 outputs (lanes other than their
+    // own seeds') in the same store instructions as the chunk's new pairs,
+    // so those partial lines fill up instead of being written twice.
+    const uint64_t nmiss = (R.noin && ((A.noin[ch] >> lane) & 1ull)) ? A.act & ~vw : 0ull;
     // Flush: per lane bit, 64 consecutive vertices' parents and levels.
-    uint64_t bits = wave_or64(got);
+    uint64_t bits = wave_or64(got | nmiss);
     // kFlush bits at a time: their parent-id loads are in flight together
     // before any store is issued.
     while (bits) {
@@ -1065,18 +1070,19 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
         lb[k] = bits ? __ffsll((unsigned long long)bits) - 1 : -1;
         bits &= bits - 1;
       }
-      bool hb[kFlush];
+      bool hb[kFlush], nb[kFlush];
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
         hb[k] = lb[k] >= 0 && ((got >> lb[k]) & 1ull);
+        nb[k] = lb[k] >= 0 && ((nmiss >> lb[k]) & 1ull);
         const int off = pt[lb[k] >= 0 ? lb[k] : 0][lane];
         pv[k] = A.in_idx[hb[k] ? rb + off : 0];  // no branch: loads go out together
       }
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
-        if (hb[k]) {
-          st_once(&so.par[lb[k]][v], pv[k]);
-          st_once(&so.lev[lb[k]][v], so.level[lb[k]]);
+        if (hb[k] || nb[k]) {
+          st_once(&so.par[lb[k]][v], hb[k] ? pv[k] : (int32_t)-1);
+          st_once(&so.lev[lb[k]][v], hb[k] ? so.level[lb[k]] : (int16_t)-1);
         }
       }
     }
@@ -1519,6 +1525,7 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   st_ctrl(&c->segtail[0], 0ull);
   st_ctrl(&c->segtail[1], 0ull);
   reset_cursors(A.cursor);
+  if (cur_pm) st_ctrl(&c->noin_left, 0);  // this round's pull visited every vertex
   // A pull rescans every row it opens from the start, so the next pull reads
   // at most what this one read: below one entry per vertex it is light.
   st_ctrl(&c->grab, cur_pm && pscanned < (long long)A.n ? kGrabSparse : 1);
@@ -1547,12 +1554,13 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   }
 }
 
-// Outputs of the vertices without in-edges: no root but the vertex itself
-// reaches them, so they are known before the search starts (-1; a seed's own
-// entry is rewritten by ms_seed_outputs afterwards) and are written on a
-// second stream while the first rounds run.  Plane-major: wavefront
-// item k is lane l = k / nch and vertices [64c, 64c + 64), c = k % nch, so
-// every output row is written front to back.
+// Outputs of the vertices without in-edges for a list of lanes: no root but
+// the vertex itself reaches them, so they are known when a lane starts (-1;
+// a seed's own entry is written afterwards).  The serving engine writes them
+// when queries take lanes; a one-call search writes them in its first pull
+// round instead (ms_pull), or in ms_finalize if no round pulls.  Plane-major:
+// wavefront item k is lane l = k / nch and vertices [64c, 64c + 64),
+// c = k % nch, so every output row is written front to back.
 // A list of lanes, passed by value.
 struct LaneList {
   int32_t count;
@@ -1593,7 +1601,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_fill_noin(const LaneOut *out, con
   }
 }
 
-// The seeds' own outputs (parent = itself, level 0), after the output fill.
+// The seeds' own outputs (parent = itself, level 0), after the last round.
 __global__ void ms_seed_outputs(MsArgs A, const int64_t *sv, const int32_t *sl, int64_t nseeds) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nseeds) return;
@@ -1608,10 +1616,11 @@ __global__ void ms_seed_outputs(MsArgs A, const int64_t *sv, const int32_t *sl, 
 __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
   const int lane = lane_id();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // vertices without in-edges were written by the first pull round, if any
+  const bool noin_done = ld_ctrl(&A.ctrl->noin_left) == 0;
   for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < A.n; v0 += stride) {
     const int64_t v = v0 + lane;
-    // vertices without in-edges are written by ms_fill_noin
-    const bool skip = v >= A.n || ((A.noin[v0 >> 6] >> lane) & 1ull);
+    const bool skip = v >= A.n || (noin_done && ((A.noin[v0 >> 6] >> lane) & 1ull));
     const uint64_t miss = skip ? 0ull : A.act & ~A.vis[v];
     uint64_t bits = wave_or64(miss);
     while (bits) {
@@ -1708,6 +1717,7 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&c->done, 1);
     st_ctrl(&c->status, 0);
     st_ctrl(&c->grab, 1);
+    st_ctrl(&c->noin_left, 0);  // retiring lanes write their own
     reset_cursors(A.cursor);
   }
 }
@@ -1910,9 +1920,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     if (r == hipSuccess) evs.push_back(*e);
     return r;
   };
-  bool side_used = false;
   auto cleanup = [&]() {
-    if (side_used) (void)hipStreamSynchronize(g->side);  // the fill may still write outputs
     for (hipEvent_t e : evs) hipEventDestroy(e);
     if (tmp) hipFree(tmp);
   };
@@ -1923,22 +1931,6 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   if (e == hipSuccess) e = new_event(&e0);
   if (e == hipSuccess) e = new_event(&e1);
   if (e == hipSuccess) e = hipEventRecord(e0, st);
-  // The outputs of vertices without in-edges go out on the side stream,
-  // overlapping the first rounds; the stream joins before ms_finalize.
-  hipEvent_t side_done = nullptr;
-  if (e == hipSuccess && !g->side) e = hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking);
-  if (e == hipSuccess) e = new_event(&side_done);
-  if (e == hipSuccess) e = hipStreamWaitEvent(g->side, e0, 0);
-  if (e == hipSuccess) {
-    LaneList all;
-    all.count = nroots;
-    for (int l = 0; l < nroots; l++) all.lane[l] = (int8_t)l;
-    ms_fill_noin<<<grid_blocks(g->device, kNoinFillPerCU), kMsBlock, 0, g->side>>>(ws.out, md->noin,
-                                                                                 g->n, all);
-    side_used = true;
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipEventRecord(side_done, g->side);
   if (e != hipSuccess) {
     cleanup();
     return hip_fail(e, "msbfs setup", __FILE__, __LINE__);
@@ -2054,12 +2046,9 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   }
   // The final pass belongs to producing the outputs, so e1 follows it.
   if (e == hipSuccess) {
-    e = hipStreamWaitEvent(st, side_done, 0);
-    if (e == hipSuccess) {
-      ms_seed_outputs<<<(unsigned)((nseeds + 255) / 256), 256, 0, st>>>(A, dsv, dsl, nseeds);
-      ms_finalize<<<blocks, kMsBlock, 0, st>>>(A);
-      e = hipGetLastError();
-    }
+    ms_seed_outputs<<<(unsigned)((nseeds + 255) / 256), 256, 0, st>>>(A, dsv, dsl, nseeds);
+    ms_finalize<<<blocks, kMsBlock, 0, st>>>(A);
+    e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipEventRecord(e1, st);
   if (e == hipSuccess) e = hipEventSynchronize(e1);

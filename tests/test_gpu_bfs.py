@@ -156,6 +156,40 @@ def test_msbfs_lanes_equal_oracle(scale, ef, seed, directed, nroots):
       _check_ms_stats(r.stats, oracle_levels, d)
 
 
+@pytest.mark.parametrize("scale,ef,seed,directed", [(12, 16, 1, False), (12, 16, 12, True)])
+def test_msbfs_every_output_written(scale, ef, seed, directed):
+  """Every (lane, vertex) output is written by the search itself: device
+  outputs pre-filled with a sentinel come back equal to the oracle under
+  every direction policy, with and without graph replay.  Vertices without
+  in-edges are written by the first pull round, or by the final pass when no
+  round pulls (topdown); roots without in-edges keep their own entries."""
+  import torch
+  abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+  g = O.Graph(n, ip, ii, op, oi, not directed)
+  outdeg, indeg = np.diff(op), np.diff(ip)
+  noin_roots = np.nonzero((outdeg > 0) & (indeg == 0))[0]
+  rng = np.random.default_rng(seed + 7)
+  roots = rng.choice(np.nonzero(outdeg > 0)[0], 64, replace=False)
+  roots[0] = int(np.argmax(outdeg))
+  roots[1:1 + min(8, len(noin_roots))] = noin_roots[:8]
+  assert not directed or len(noin_roots) > 0
+  expect = [O.bfs(g, int(s)) for s in roots]
+  par = torch.empty(64 * n, dtype=torch.int32, device="cuda")
+  lev = torch.empty(64 * n, dtype=torch.int16, device="cuda")
+  with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
+    for d, cap in [(d, False) for d in DIRECTIONS + ["beamer_lanes"]] + [("beamer", True)]:
+      par.fill_(0x5A5A5A5A)
+      lev.fill_(0x5A5A)
+      dg.msbfs(roots, direction=d, capture=cap, device_out=(par, lev))
+      P = par.view(64, n).cpu().numpy()
+      L = lev.view(64, n).cpu().numpy()
+      for l in range(64):
+        lev0, par0, _, _ = expect[l]
+        np.testing.assert_array_equal(L[l], lev0.astype(np.int16), err_msg=f"lane {l} dir {d} {cap}")
+        np.testing.assert_array_equal(P[l], par0, err_msg=f"lane {l} dir {d} {cap}")
+
+
 def test_msbfs_edge_cases():
   """Roots without in-edges, isolated roots, the same root in several lanes,
   a hub above the segment length, a deep path, n not a multiple of 64."""
