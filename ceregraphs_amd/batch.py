@@ -46,7 +46,7 @@ UNEVEN_ROOT_SEED = 18
 UNEVEN_ROOTS_PER_GRAPH = 32
 UNEVEN_BALANCE_GRID = 4  # P of the P x P balancing grid (0: generator ids)
 
-DEFAULT_CONFIG = {"direction": "beamer", "alpha": 14.0, "beta": 24.0, "grid_mult": 8}
+DEFAULT_CONFIG = {"direction": "beamer", "alpha": 14.0, "beta": 24.0, "grid_mult": 0}  # 0: by graph size
 
 
 def union_csr(parts):

@@ -613,7 +613,12 @@ void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o) {
   if (o->beta <= 0) o->beta = 24.0;
   if (o->tau <= 0) o->tau = (int64_t)(0.15 * (double)n + 0.5);
   if (o->max_levels <= 0) o->max_levels = n + 1;
-  if (o->grid_mult <= 0) o->grid_mult = 8;
+  // Blocks per CU of the grid-stride kernels (and the cap of the pull's
+  // resident grid).  Measured on R-MAT ef16 graphs, single-root and 64-root
+  // searches alike: 2 is best up to 2^18 vertices, 4 from 2^19 to 2^22, 8
+  // from 2^23 (short rounds on small graphs pay for every extra block;
+  // large graphs need the full grid).
+  if (o->grid_mult <= 0) o->grid_mult = n < (1 << 19) ? 2 : n < (1 << 23) ? 4 : 8;
 }
 
 }  // namespace cg

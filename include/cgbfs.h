@@ -63,7 +63,8 @@ typedef struct {
   double beta;        /* pull->push when n_f < n / beta               (24)   */
   int64_t tau;        /* CGBFS_DIR_TAU threshold on nf                (0.15n)*/
   int32_t block;      /* threads per block for level kernels          (256)  */
-  int32_t grid_mult;  /* blocks per CU for grid-stride kernels        (8)    */
+  int32_t grid_mult;  /* blocks per CU for grid-stride kernels (2 below 2^19
+                         vertices, 4 below 2^23, else 8)                    */
   int64_t max_levels; /* abort with CGBFS_ETIMEOUT beyond this        (n+1)  */
   int32_t capture;    /* multi-source: capture the round kernels once into a
                          HIP graph and replay it, deciding every level on the
