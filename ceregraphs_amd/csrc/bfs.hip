@@ -34,6 +34,8 @@ constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/abov
 constexpr int kBigLds = 4096;         // push: big-list prefix entries staged per pass
 constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this
 constexpr int64_t kPullWaveDegree = 128;  // pull: wavefront scan at/above this
+constexpr int kPullUnroll = 4;         // pull: row entries a lane tests per step
+constexpr int32_t kPullGrab = 4;       // pull: chunks per grab from the cursors
 constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
 
 struct Ctrl {
@@ -47,8 +49,10 @@ struct Ctrl {
   unsigned long long rows;      // adjacency lists opened this round
   long long mu;                 // in-edges of still-unvisited vertices
   long long prev_nf;
+  long long seen;               // vertices reached so far
   long long rounds;
-  int dir;  // direction of the current round: 0 push, 1 pull
+  int dir;   // direction of the current round: 0 push, 1 pull
+  int light; // the current round's pull splits its chunks statically
   int done;
   int status;
   unsigned long long t0;
@@ -60,6 +64,7 @@ struct BfsParams {
   int mode;
   double alpha, beta;
   long long tau, max_levels;
+  long long n_in;  // vertices with at least one in-edge
 };
 
 struct Hist {
@@ -83,6 +88,7 @@ struct Workspace {
   Ctrl *ctrl = nullptr;
   Hist *hist = nullptr;
   unsigned long long *scalars = nullptr;  // m / visited reductions
+  unsigned long long *cursor = nullptr;   // [kParts * kCursorStride] pull chunk cursors
 };
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -93,7 +99,8 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
   size_t bm = align_up(sizeof(uint32_t) * nwords);
   size_t qb = align_up(sizeof(int32_t) * n);
   size_t total = bm * 4 + qb * 3 + align_up(sizeof(int64_t) * (n + 1)) + align_up(sizeof(int32_t) * n) + align_up(sizeof(uint32_t) * n) +
-                 align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64);
+                 align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64) +
+                 align_up(sizeof(unsigned long long) * kParts * kCursorStride);
   if (g->ws1_bytes < total) {
     if (g->ws1) hipFree(g->ws1);
     g->ws1 = nullptr;
@@ -113,7 +120,8 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
   w->parents = (uint32_t *)p; p += align_up(sizeof(uint32_t) * n);
   w->ctrl = (Ctrl *)p; p += align_up(sizeof(Ctrl));
   w->hist = (Hist *)p; p += align_up(sizeof(Hist));
-  w->scalars = (unsigned long long *)p;
+  w->scalars = (unsigned long long *)p; p += align_up(64);
+  w->cursor = (unsigned long long *)p;
   return CGBFS_OK;
 }
 
@@ -147,6 +155,7 @@ struct LevelArgs {
   int64_t *bigpre;
   Ctrl *ctrl;
   Hist *hist;
+  unsigned long long *cursor;  // pull chunk cursors, zero between rounds
   int64_t n;
   int64_t nwords;
 };
@@ -187,11 +196,14 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, BfsParams P, long long t
   st_ctrl(&c->rows, 0ull);
   st_ctrl(&c->mu, (long long)(total_in - (A.in_ptr[s + 1] - A.in_ptr[s])));
   st_ctrl(&c->prev_nf, 1ll);
+  st_ctrl(&c->seen, 1ll);
+  st_ctrl(&c->light, 0);
   st_ctrl(&c->rounds, 0ll);
   // round 0 is push (bool_pe.csl:714-719) unless every round pulls
   st_ctrl(&c->dir, P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0);
   st_ctrl(&c->done, 0);
   st_ctrl(&c->status, 0);
+  reset_cursors(A.cursor);
   const unsigned long long t0 = wall_clock64();
   st_ctrl(&c->t0, t0);
   st_ctrl(&c->t_end, t0);
@@ -429,30 +441,41 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
   for (int64_t w = gtid; w < A.nwords; w += gstride) fclear[w] = 0;
 
   const int lane = lane_id();
-  const int64_t wave = gtid / kWave;
-  const int64_t nwaves = gstride / kWave;
   const int64_t nchunks = A.nwords / 2;
   unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0;
-  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
+  // One chunk of 64 vertices (wavefront-uniform).
+  auto chunk = [&](int64_t ch) {
     const int64_t w0 = ch * 2;
     uint32_t vlo = A.visited[w0], vhi = A.visited[w0 + 1];
     const int64_t v = ch * kWave + lane;
     uint32_t myword = (lane < 32) ? vlo : vhi;
     bool todo = v < A.n && !((myword >> (lane & 31)) & 1u);
-    if (!__any(todo)) continue;
+    if (!__any(todo)) return;
     int64_t beg = todo ? A.in_ptr[v] : 0, end = todo ? A.in_ptr[v + 1] : 0;
     bool longrow = todo && (end - beg) >= kPullWaveDegree;
     bool hit = false;
     int32_t par = -1;
     if (todo) rows++;
     if (todo && !longrow) {
-      for (int64_t e = beg; e < end; e++) {
-        int32_t u = A.in_idx[e];
-        scanned++;
-        if ((fcur[u >> 5] >> (u & 31)) & 1u) {
-          hit = true;
-          par = u;
-          break;
+      // kPullUnroll entries per step: their index loads, then their
+      // frontier-bit loads, go out together; the first hit in row order is
+      // still the lowest id.  Lanes past the row end reread its first entry.
+      for (int64_t e0 = beg; e0 < end && !hit; e0 += kPullUnroll) {
+        int32_t u[kPullUnroll];
+        uint32_t fw[kPullUnroll];
+#pragma unroll
+        for (int k = 0; k < kPullUnroll; k++) u[k] = A.in_idx[e0 + k < end ? e0 + k : e0];
+#pragma unroll
+        for (int k = 0; k < kPullUnroll; k++) fw[k] = fcur[u[k] >> 5];
+#pragma unroll
+        for (int k = 0; k < kPullUnroll; k++) {
+          if (!hit && e0 + k < end) {
+            scanned++;
+            if ((fw[k] >> (u[k] & 31)) & 1u) {
+              hit = true;
+              par = u[k];
+            }
+          }
         }
       }
     }
@@ -501,7 +524,15 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
       }
       sq.push(hit, (int32_t)v);
     }
-  }
+  };
+  // Chunks come from the cursors: their costs differ by orders of magnitude
+  // (rows scanned to the end, hub rows, chunks already visited), which a
+  // static split over the wavefronts leaves unbalanced.  Once few vertices
+  // with in-edges are unreached, nearly every chunk is visited and costs
+  // less than the atomic that would hand it out: a static split then.
+  ChunkFeed feed(A.cursor, (int32_t)nchunks, kPullGrab, ld_ctrl(&c->light) != 0);
+  int32_t cb = 0, ce = 0;
+  while (cb < ce || feed.next(cb, ce)) chunk(cb++);
   sq.flush();
   unsigned long long s;
   s = block_sum(mf, red);
@@ -544,6 +575,7 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
   st_ctrl(&c->in_sum, 0ull);
   st_ctrl(&c->scanned, 0ull);
   st_ctrl(&c->rows, 0ull);
+  reset_cursors(A.cursor);
   if (nf == 0) {
     st_ctrl(&c->done, 1);
     return;
@@ -563,6 +595,11 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
       }
   }
   st_ctrl(&c->prev_nf, nf);
+  // The next pull visits every chunk; once fewer vertices with in-edges are
+  // unreached than there are chunks, most chunks have nothing to do.
+  const long long seen = ld_ctrl(&c->seen) + nf;
+  st_ctrl(&c->seen, seen);
+  st_ctrl(&c->light, P.n_in - seen < (long long)(A.nwords / 2) ? 1 : 0);
   st_ctrl(&c->dir, dir);
   if (d + 1 >= P.max_levels) {
     st_ctrl(&c->status, (int)CGBFS_ETIMEOUT);
@@ -656,6 +693,7 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   A.bigpre = ws.bigpre;
   A.ctrl = ws.ctrl;
   A.hist = ws.hist;
+  A.cursor = ws.cursor;
   A.n = g->n;
   A.nwords = ws.nwords;
 
@@ -672,6 +710,7 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   P.beta = o.beta;
   P.tau = (long long)o.tau;
   P.max_levels = (long long)o.max_levels;
+  P.n_in = (long long)g->n_in;
   bfs_init_source<<<1, 64, 0, st>>>(A, source, P, (long long)g->total_out_edges);
   CG_HIP(hipGetLastError());
 

@@ -52,8 +52,8 @@ struct DeviceGraph {
   // bfs/implementation/bfs_timing.py:390-395.
   int32_t *m_weight = nullptr;
   int64_t total_out_edges = 0;
+  int64_t n_in = 0;  // vertices with at least one in-edge
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;  // second stream, for work that overlaps a search
   // Lazily sized single-root workspace.
   void *ws1 = nullptr;
   size_t ws1_bytes = 0;
@@ -122,6 +122,85 @@ __device__ __forceinline__ void st_once(T *p, T v) {
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// A pull hands out its 64-vertex chunks dynamically from kParts cursors, one
+// per contiguous part of the vertex range, kept a cache line apart.
+constexpr int kParts = 8;
+constexpr int kCursorStride = 16;
+
+// Zeroes the pull's chunk cursors for the next round (one thread).
+__device__ __forceinline__ void reset_cursors(unsigned long long *cur) {
+  for (int p = 0; p < kParts; ++p) st_ctrl(&cur[p * kCursorStride], 0ull);
+}
+
+// Hands out a pull's 64-vertex chunks (the single-root and the multi-source
+// pulls).  Part p holds chunks [nch*p/kParts, nch*(p+1)/kParts).  A
+// wavefront draws from its home part first, then from the other parts in
+// turn, so the parts finish together whatever their costs.  A grab is one
+// chunk in rounds with real scanning (a dense round has only a few dozen
+// chunks per wavefront, each costing hundreds of microseconds, and needs
+// single chunks, taken only when the previous one is done, to balance) and
+// several chunks in a round known to be light, where one atomic per chunk
+// would cost more than the chunk; there the next grab is issued before the
+// current chunks are worked on, so its latency hides behind theirs.  The
+// round's decide kernel picks the size.  Blocks reach the XCDs round-robin,
+// so with home = blockIdx.x % kParts each XCD mostly walks one contiguous
+// stretch of vertices.  Wavefront-uniform.
+struct ChunkFeed {
+  unsigned long long *cur;
+  int32_t nch;
+  int home;
+  int32_t want;    // chunks per grab
+  int tried = 0;   // parts found drained so far
+  bool pending = false;  // a grab is in flight
+  int32_t issued = 0;    // lane 0: the cursor value it returned
+  int32_t size = 0;      // chunks the grab in flight asked for
+  // Static split (a round known to have little to do, where most chunks
+  // cost less than the atomic that would hand them out): chunk pos, then
+  // pos + stride, ..., one wavefront of the grid each.
+  bool fixed;
+  int32_t pos = 0, stride = 0;
+
+  __device__ ChunkFeed(unsigned long long *c, int32_t n, int32_t grab, bool static_split)
+      : cur(c), nch(n), home((int)(blockIdx.x % kParts)), want(grab), fixed(static_split) {
+    if (fixed) {
+      pos = (int32_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+      stride = (int32_t)(((int64_t)gridDim.x * blockDim.x) / kWave);
+    }
+  }
+
+  __device__ int part() const { return (home + tried) % kParts; }
+  __device__ void issue() {
+    if (lane_id() == 0)
+      issued = (int32_t)atomicAdd(&cur[part() * kCursorStride], (unsigned long long)want);
+    size = want;
+    pending = true;
+  }
+  // The next chunks, [b, e); false once every part is drained.
+  __device__ bool next(int32_t &b, int32_t &e) {
+    if (fixed) {
+      if (pos >= nch) return false;
+      b = pos;
+      e = pos + 1;
+      pos += stride;
+      return true;
+    }
+    for (;;) {
+      if (!pending) issue();
+      pending = false;
+      const int p = part();
+      const int64_t lo = (int64_t)nch * p / kParts, hi = (int64_t)nch * (p + 1) / kParts;
+      const int64_t k = lo + __shfl(issued, 0);
+      if (k < hi) {
+        b = (int32_t)k;
+        e = (int32_t)(k + size < hi ? k + size : hi);
+        if (want > 1) issue();
+        return true;
+      }
+      if (++tried == kParts) return false;
+    }
+  }
+};
 
 // Appends `count` items (wave-aggregated) to a queue; returns the base index
 // for this lane's item, valid only where `want` is true.

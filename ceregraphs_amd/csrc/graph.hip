@@ -202,6 +202,15 @@ __global__ void m_weight_kernel(const int64_t *__restrict__ in_ptr,
   w[v] = (int32_t)(lo - b);
 }
 
+// Counts the vertices with at least one in-edge (one atomic per wavefront).
+__global__ void count_in_kernel(const int64_t *__restrict__ in_ptr, int64_t n,
+                                unsigned long long *__restrict__ count) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool has = v < n && in_ptr[v + 1] > in_ptr[v];
+  const unsigned long long c = (unsigned long long)__popcll(__ballot(has));
+  if (lane_id() == 0 && c) atomicAdd(count, c);
+}
+
 static int sort_unique_to_csr(uint64_t *keys, uint64_t *scratch, int64_t count, int S, int64_t n,
                               hipStream_t stream, int64_t **ptr_out, int32_t **idx_out,
                               int64_t *nk_out) {
@@ -256,9 +265,21 @@ int graph_finalize(DeviceGraph *g) {
   }
   CG_HIP(hipHostMalloc((void **)&g->host_flag, 64, hipHostMallocDefault));
   int64_t last = 0;
-  CG_HIP(hipMemcpyAsync(&last, g->out_ptr + g->n, sizeof(int64_t), hipMemcpyDeviceToHost, g->stream));
-  CG_HIP(hipStreamSynchronize(g->stream));
+  unsigned long long *cnt = nullptr, n_in = 0;
+  CG_HIP(hipMalloc(&cnt, sizeof(unsigned long long)));
+  hipError_t e = hipMemsetAsync(cnt, 0, sizeof(unsigned long long), g->stream);
+  if (e == hipSuccess && g->n > 0) {
+    count_in_kernel<<<(unsigned)((g->n + 255) / 256), 256, 0, g->stream>>>(g->in_ptr, g->n, cnt);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&n_in, cnt, sizeof(n_in), hipMemcpyDeviceToHost, g->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&last, g->out_ptr + g->n, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                          g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  hipFree(cnt);
+  CG_HIP(e);
   g->total_out_edges = last;
+  g->n_in = (int64_t)n_in;
   return CGBFS_OK;
 }
 
@@ -276,7 +297,6 @@ static void graph_free(DeviceGraph *g) {
   if (g->msdata && g->msdata_free) g->msdata_free(g->msdata);
   if (g->host_flag) hipHostFree(g->host_flag);
   if (g->stream) hipStreamDestroy(g->stream);
-  if (g->side) hipStreamDestroy(g->side);
   // DeviceGraph is the first and only member of the handle.
   delete reinterpret_cast<cgbfs_graph *>(g);
 }

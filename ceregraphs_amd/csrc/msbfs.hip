@@ -57,11 +57,7 @@ constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
 constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
-// The pull hands out its 64-vertex chunks dynamically from kParts cursors,
-// one per contiguous part of the vertex range, kept a cache line apart.
-constexpr int kParts = 8;
-constexpr int kCursorStride = 16;
-constexpr int32_t kGrabSparse = 8;  // chunks per pull grab when the pull is known to be light
+constexpr int32_t kGrabSparse = 8;  // marks a pull known to be light: it splits its chunks statically
 // First-hit offsets within a wavefront-tier row (< kSeg) fit in a byte.
 using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
@@ -716,57 +712,6 @@ __global__ void __launch_bounds__(kMsBlock) ms_init_fill(MsArgs A) {
   for (int64_t i = tid; i < A.pin.nbig; i += stride) A.bgot[i] = 0;
 }
 
-// Zeroes the pull's chunk cursors for the next round (one thread).
-__device__ __forceinline__ void reset_cursors(unsigned long long *cur) {
-  for (int p = 0; p < kParts; ++p) st_ctrl(&cur[p * kCursorStride], 0ull);
-}
-
-// Hands out the pull's 64-vertex chunks.  Part p holds chunks
-// [nch*p/kParts, nch*(p+1)/kParts).  A wavefront draws from its home part
-// first, then from the other parts in turn, so the parts finish together
-// whatever their costs.  A grab is one chunk in rounds with real scanning (a
-// dense round has only a few dozen chunks per wavefront, each costing
-// hundreds of microseconds, and needs single chunks, taken only when the
-// previous one is done, to balance) and kGrabSparse chunks in a round known
-// to be light, where one atomic per chunk would cost more than the chunk;
-// there the next grab is issued before the current chunks are worked on, so
-// its latency hides behind theirs.  ms_decide picks the size.  Blocks reach
-// the XCDs round-robin, so with home = blockIdx.x % kParts each XCD mostly
-// walks one contiguous stretch of vertices.  Wavefront-uniform.
-struct ChunkFeed {
-  unsigned long long *cur;
-  int32_t nch;
-  int home;
-  int32_t want;    // chunks per grab
-  int tried;       // parts found drained so far
-  bool pending;    // a grab is in flight
-  int32_t issued;  // lane 0: the cursor value it returned
-
-  __device__ int part() const { return (home + tried) % kParts; }
-  __device__ void issue() {
-    if (lane_id() == 0)
-      issued = (int32_t)atomicAdd(&cur[part() * kCursorStride], (unsigned long long)want);
-    pending = true;
-  }
-  // The next chunks, [b, e); false once every part is drained.
-  __device__ bool next(int32_t &b, int32_t &e) {
-    for (;;) {
-      if (!pending) issue();
-      pending = false;
-      const int p = part();
-      const int64_t lo = (int64_t)nch * p / kParts, hi = (int64_t)nch * (p + 1) / kParts;
-      const int64_t k = lo + __shfl(issued, 0);
-      if (k < hi) {
-        b = (int32_t)k;
-        e = (int32_t)(k + want < hi ? k + want : hi);
-        if (want > 1) issue();
-        return true;
-      }
-      if (++tried == kParts) return false;
-    }
-  }
-};
-
 __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
   RoundState r;
   r.d = (int)ld_ctrl(&c->rounds);
@@ -947,7 +892,7 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
   // Chunks come from the cursors: their costs differ by orders of magnitude
   // (row lengths, how soon rows are satisfied), which a static split of the
   // chunks over the wavefronts leaves unbalanced.
-  ChunkFeed feed{A.cursor, nch, (int)(blockIdx.x % kParts), R.grab, 0, false, 0};
+  ChunkFeed feed(A.cursor, nch, R.grab, R.grab == kGrabSparse);
   int32_t cb = 0, ce = 0;
   while (cb < ce || feed.next(cb, ce)) {
     const int32_t ch = cb++;
@@ -1527,7 +1472,8 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   reset_cursors(A.cursor);
   if (cur_pm) st_ctrl(&c->noin_left, 0);  // this round's pull visited every vertex
   // A pull rescans every row it opens from the start, so the next pull reads
-  // at most what this one read: below one entry per vertex it is light.
+  // at most what this one read: below one entry per vertex it is light, and
+  // its chunks, most of which have nothing to scan, are split statically.
   st_ctrl(&c->grab, cur_pm && pscanned < (long long)A.n ? kGrabSparse : 1);
   // A pull part lists none of its new vertices: the next round's frontier is
   // listed only if this round did not pull, and a push part after a pull
