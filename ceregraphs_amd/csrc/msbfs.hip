@@ -1005,6 +1005,11 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
     const uint64_t nmiss = (R.noin && ((A.noin[ch] >> lane) & 1ull)) ? A.act & ~vw : 0ull;
     // Flush: per lane bit, 64 consecutive vertices' parents and levels.
     uint64_t bits = wave_or64(got | nmiss);
+    // First hits are the lowest ids with the bit, mostly near the row start.
+    constexpr int kHead = 4;
+    int32_t head[kHead];
+#pragma unroll
+    for (int j = 0; j < kHead; j++) head[j] = A.in_idx[got && j < len ? rb + j : 0];
     // kFlush bits at a time: their parent-id loads are in flight together
     // before any store is issued.
     while (bits) {
@@ -1021,7 +1026,10 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
         hb[k] = lb[k] >= 0 && ((got >> lb[k]) & 1ull);
         nb[k] = lb[k] >= 0 && ((nmiss >> lb[k]) & 1ull);
         const int off = pt[lb[k] >= 0 ? lb[k] : 0][lane];
-        pv[k] = A.in_idx[hb[k] ? rb + off : 0];  // no branch: loads go out together
+        // no branch: loads go out together; a hit among the row's first
+        // kHead entries takes its id from the row head loaded once per chunk
+        const int32_t far = A.in_idx[hb[k] && off >= kHead ? rb + off : 0];
+        pv[k] = off == 0 ? head[0] : off == 1 ? head[1] : off == 2 ? head[2] : off == 3 ? head[3] : far;
       }
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
