@@ -124,9 +124,9 @@ def main():
   batched_aggregate = m.sum() / t_step / 1e9
   lane_hmean = harmonic_mean(m / t_step / 1e9)
 
-  # Roofline of the dominant kernel: the multi-source pull (ms_pull and its
-  # big-vertex epilogue ms_pull_fin), timed by the library's HIP events on
-  # the graph's stream.
+  # Roofline of the dominant kernel: the multi-source pull (ms_pull_segs over
+  # the rows of degree >= 256, then ms_pull over the 64-vertex chunks), timed
+  # by the library's HIP events on the graph's stream.
   st = last.stats
   for i in range(int(st.rounds_completed)):
     log(f"  round {i}: dir {st.direction_history[i]} pull lanes {st.pull_lanes[i]:2d} "
@@ -149,7 +149,7 @@ def main():
       traffic = json.load(f).get("hbm_bytes_per_launch")
   roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
               "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-              "kernel": "ms_pull+ms_pull_fin", "launches": npull,
+              "kernel": "ms_pull_segs+ms_pull", "launches": npull,
               "avg_launch_ms": round(tpull * 1e3 / max(npull, 1), 4),
               "algorithmic_bytes_per_launch": int(byts / max(npull, 1))}
   log(f"pull rounds: {npull}, {tpull*1e3:.3f} ms, {byts/1e9:.2f} GB algorithmic -> "

@@ -24,7 +24,7 @@
 // to a hub is a level ahead of a leaf root), so one direction for all lanes
 // would make the pull rescan whole adjacency lists for lanes that are not
 // there yet.  Lanes with an empty frontier take no part.
-// Pull part (ms_pull + ms_pull_fin): every vertex with an unreached pull
+// Pull part (ms_pull_segs + ms_pull): every vertex with an unreached pull
 // lane ORs the frontier words of its ascending in-neighbours; within a step
 // of 64 neighbours an exclusive prefix-OR across the wavefront gives each
 // lane of each vertex its first (lowest-id) hit.  Stops once every needed
@@ -152,6 +152,7 @@ struct SegList {
   int32_t *big_v = nullptr;    // big vertices, by index
   int32_t *seg_first = nullptr; // [nbig + 1] first segment of each big vertex
   uint8_t *is_big = nullptr;   // per vertex
+  int32_t *chunk_big = nullptr; // [n/64 + 1] big vertices below each 64-vertex chunk
 };
 
 struct MsArgs {
@@ -175,7 +176,9 @@ struct MsArgs {
   SegList pin, pout;
   int32_t *segq[2];
   uint64_t *segneed;  // [pin.nseg] lanes needed by each active pull segment
-  unsigned long long *cursor;  // [kParts * kCursorStride] pull chunk cursors, zero between rounds
+  // [2 * kParts * kCursorStride] pull cursors, zero between rounds: chunks,
+  // then active big-vertex segments
+  unsigned long long *cursor;
   const uint64_t *noin;  // vertices without in-edges, one bit each
   int *host_done;        // pinned host flag raised with done (graph replay), or null
   // Vertex-state slot of each vertex (hub-first order) and the adjacency
@@ -235,9 +238,12 @@ struct MsWorkspace {
 // ---------------------------------------------------------------------------
 static void seg_plan(const std::vector<int64_t> &ptr, int64_t n, std::vector<int32_t> &sv,
                      std::vector<int32_t> &sb, std::vector<int64_t> &sg, std::vector<int32_t> &bv,
-                     std::vector<int32_t> &bf, std::vector<uint8_t> &isb) {
+                     std::vector<int32_t> &bf, std::vector<uint8_t> &isb,
+                     std::vector<int32_t> &cbig) {
   isb.assign(n, 0);
+  cbig.assign((size_t)((n + 63) / 64 + 1), 0);
   for (int64_t v = 0; v < n; v++) {
+    if ((v & 63) == 0) cbig[(size_t)(v >> 6)] = (int32_t)bv.size();  // big_v ascends by vertex
     int64_t deg = ptr[v + 1] - ptr[v];
     if (deg < kSeg) continue;
     int32_t bid = (int32_t)bv.size();
@@ -251,6 +257,7 @@ static void seg_plan(const std::vector<int64_t> &ptr, int64_t n, std::vector<int
     }
   }
   bf.push_back((int32_t)sv.size());
+  cbig.back() = (int32_t)bv.size();
 }
 
 static int build_seglists(DeviceGraph *g, MsGraphData *md) {
@@ -260,12 +267,13 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
   std::vector<int32_t> sv[2], sb[2], bv[2], bf[2];
   std::vector<int64_t> sg[2];
   std::vector<uint8_t> isb[2];
-  seg_plan(ptr_in, n, sv[0], sb[0], sg[0], bv[0], bf[0], isb[0]);
+  std::vector<int32_t> cbig[2];
+  seg_plan(ptr_in, n, sv[0], sb[0], sg[0], bv[0], bf[0], isb[0], cbig[0]);
   const bool shared = g->in_ptr == g->out_ptr;
   if (!shared) {
     ptr_out.resize(n + 1);
     CG_HIP(hipMemcpy(ptr_out.data(), g->out_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
-    seg_plan(ptr_out, n, sv[1], sb[1], sg[1], bv[1], bf[1], isb[1]);
+    seg_plan(ptr_out, n, sv[1], sb[1], sg[1], bv[1], bf[1], isb[1], cbig[1]);
   }
   std::vector<int32_t> sl[2];
   for (int k = 0; k < (shared ? 1 : 2); k++) {
@@ -279,7 +287,8 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
   for (int k = 0; k < views; k++)
     total += al(sv[k].size() * 4 + 4) + al(sb[k].size() * 4 + 4) + al(sg[k].size() * 8 + 8) +
              al(sl[k].size() * 4 + 4) +
-             al(bv[k].size() * 4 + 4) + al(bf[k].size() * 4) + al((size_t)n);
+             al(bv[k].size() * 4 + 4) + al(bf[k].size() * 4) + al((size_t)n) +
+             al(cbig[k].size() * 4);
   CG_HIP(hipMalloc(&md->mem, total));
   char *p = (char *)md->mem;
   for (int k = 0; k < views; k++) {
@@ -293,6 +302,7 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
     L.big_v = (int32_t *)p; p += al(bv[k].size() * 4 + 4);
     L.seg_first = (int32_t *)p; p += al(bf[k].size() * 4);
     L.is_big = (uint8_t *)p; p += al((size_t)n);
+    L.chunk_big = (int32_t *)p; p += al(cbig[k].size() * 4);
     if (L.nseg) {
       CG_HIP(hipMemcpy(L.seg_v, sv[k].data(), sv[k].size() * 4, hipMemcpyHostToDevice));
       CG_HIP(hipMemcpy(L.seg_bid, sb[k].data(), sb[k].size() * 4, hipMemcpyHostToDevice));
@@ -302,6 +312,7 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
     }
     CG_HIP(hipMemcpy(L.seg_first, bf[k].data(), bf[k].size() * 4, hipMemcpyHostToDevice));
     CG_HIP(hipMemcpy(L.is_big, isb[k].data(), (size_t)n, hipMemcpyHostToDevice));
+    CG_HIP(hipMemcpy(L.chunk_big, cbig[k].data(), cbig[k].size() * 4, hipMemcpyHostToDevice));
   }
   if (shared) md->pout = md->pin;
   {
@@ -395,6 +406,12 @@ static void free_msdata(void *q) {
   delete m;
 }
 
+// Both sets of pull cursors (chunks, then big-vertex segments) back to zero.
+__device__ __forceinline__ void reset_ms_cursors(unsigned long long *cur) {
+  reset_cursors(cur);
+  reset_cursors(cur + kParts * kCursorStride);
+}
+
 // Lays a multi-source workspace out from `base` (sizes only when base is
 // null) and returns its total size.
 static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspace *w) {
@@ -406,7 +423,7 @@ static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspac
       al(sizeof(LaneOut)), al(sizeof(unsigned long long) * 64),
       al(sizeof(uint32_t) * 64 * (nbig + 1)), al(sizeof(uint64_t) * (nbig + 1)),
       al(sizeof(int32_t) * (md->pin.nseg + 1)), al(sizeof(int32_t) * (md->pout.nseg + 1)),
-      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(unsigned long long) * kParts * kCursorStride)};
+      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(unsigned long long) * 2 * kParts * kCursorStride)};
   size_t total = 0;
   for (size_t x : sizes) total += x;
   if (!base) return total;
@@ -660,7 +677,7 @@ __device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t 
                                                 int64_t end, uint64_t need, uint32_t *cand,
                                                 bool use_fb, uint32_t &scanned) {
   const int lane = lane_id();
-  int32_t sl[kSegSteps], uid[kSegSteps];
+  int32_t sl[kSegSteps];
   bool ok[kSegSteps];
   uint64_t w[kSegSteps];
 #pragma unroll
@@ -669,7 +686,6 @@ __device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t 
     ok[k] = e < end;
     const int64_t ee = ok[k] ? e : beg;
     sl[k] = A.in_slot[ee];
-    uid[k] = A.in_idx[ee];
   }
   if (use_fb) {
 #pragma unroll
@@ -686,10 +702,13 @@ __device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t 
     const uint64_t inc = wave_scan_or(h);
     const uint64_t before = lane_before(inc);
     uint64_t mine = h & ~before;
-    while (mine) {
-      int l = __ffsll((unsigned long long)mine) - 1;
-      mine &= mine - 1;
-      atomicMin(&cand[l], (uint32_t)uid[k]);
+    if (mine) {  // the id is read only for a first hit (h != 0: the entry is in range)
+      const uint32_t u = (uint32_t)A.in_idx[beg + k * kWave + lane];
+      while (mine) {
+        int l = __ffsll((unsigned long long)mine) - 1;
+        mine &= mine - 1;
+        atomicMin(&cand[l], u);
+      }
     }
     got |= shfl64(inc, kWave - 1);
     if (got == need) break;
@@ -794,7 +813,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->status, 0);
     st_ctrl(&c->grab, 1);
     st_ctrl(&c->noin_left, 1);
-    reset_cursors(A.cursor);
+    reset_ms_cursors(A.cursor);
   }
   bool live = false;
   int dir = 0;
@@ -871,8 +890,6 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
   uint32_t scanned = 0, rows = 0, pairs = 0, mf = 0, nf = 0;
   uint64_t live = 0;
   const int lane = lane_id();
-  const int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-  const int32_t nwaves = (int32_t)((gridDim.x * blockDim.x) / kWave);
   // First hits of the current chunk, as offsets into the row (rows here are
   // shorter than kSeg), indexed [lane bit][row] so the per-bit flush below
   // reads one conflict-free row of LDS and stores 64 consecutive vertices.
@@ -907,6 +924,13 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
     const uint64_t vw = A.vis[vv];
     const uint64_t need = (inr && !big) ? pm & ~vw : 0ull;
     const int64_t rb = rb0;
+    // A big vertex's row was pulled by segments (ms_pull_segs, before this
+    // kernel): its new lanes and their parents wait in bgot / cand, indexed
+    // by its rank among the big vertices (they ascend by vertex id).
+    const uint64_t bigm = ballot64(inr && big);
+    int32_t bid = 0;
+    if (bigm)
+      bid = A.pin.chunk_big[ch] + (int32_t)__popcll(bigm & ((1ull << lane) - 1ull));
     const int32_t len = need ? (int32_t)(re0 - rb0) : 0;  // < kSeg for non-big rows
     if (len) rows++;
     // exclusive prefix sum of row lengths -> virtual starts
@@ -1003,6 +1027,10 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
     // own seeds') in the same store instructions as the chunk's new pairs,
     // so those partial lines fill up instead of being written twice.
     const uint64_t nmiss = (R.noin && ((A.noin[ch] >> lane) & 1ull)) ? A.act & ~vw : 0ull;
+    const bool bg = (bigm >> lane) & 1ull;  // this lane's vertex is big
+    if (bg) got = A.bgot[bid];
+    // big vertices with new lanes (rare after the round that reaches the hubs)
+    const bool bgot_any = __any(bg && got);
     // Flush: per lane bit, 64 consecutive vertices' parents and levels.
     uint64_t bits = wave_or64(got | nmiss);
     // First hits are the lowest ids with the bit, mostly near the row start.
@@ -1028,8 +1056,12 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
         const int off = pt[lb[k] >= 0 ? lb[k] : 0][lane];
         // no branch: loads go out together; a hit among the row's first
         // kHead entries takes its id from the row head loaded once per chunk
-        const int32_t far = A.in_idx[hb[k] && off >= kHead ? rb + off : 0];
+        const int32_t far = A.in_idx[hb[k] && !bg && off >= kHead ? rb + off : 0];
         pv[k] = off == 0 ? head[0] : off == 1 ? head[1] : off == 2 ? head[2] : off == 3 ? head[3] : far;
+        if (bgot_any) {  // a big vertex's parent is its candidate
+          const uint32_t cv = A.cand[bg && hb[k] ? (int64_t)bid * 64 + lb[k] : 0];
+          if (bg) pv[k] = (int32_t)cv;
+        }
       }
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
@@ -1039,29 +1071,24 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
         }
       }
     }
+    if (bg && got) {  // candidates back to +inf for the next pull
+      uint4 *crow = reinterpret_cast<uint4 *>(&A.cand[(int64_t)bid * 64]);
+#pragma unroll
+      for (int j = 0; j < 16; j++) crow[j] = make_uint4(~0u, ~0u, ~0u, ~0u);
+      A.bgot[bid] = 0;
+    }
+    const int64_t indeg = re0 - rb0;
     int64_t od = 0;
     if (got) {
       nx[sv_me] = got;
       A.vis[v] = vw | got;
-      od = sym ? (int64_t)len : A.out_ptr[v + 1] - A.out_ptr[v];
+      od = sym ? indeg : A.out_ptr[v + 1] - A.out_ptr[v];
       mf += (uint32_t)od;
       live |= got;
       pairs += (uint32_t)__popcll(got);
       nf++;
     }
-    if (tally_on) tally_words(got, clamp_deg(od), clamp_deg(len), sym, tally);
-  }
-  // Big vertices: one active segment per wavefront; the record carries the
-  // lanes its vertex needs (vis of big vertices changes only in ms_pull_fin).
-  const int64_t nact = (int64_t)ld_ctrl(&c->segtail[0]);
-  for (int64_t i = wave; i < nact; i += nwaves) {
-    const int32_t s = A.segq[0][i];
-    const uint64_t need = A.segneed[i];
-    const int32_t bid = A.pin.seg_bid[s];
-    const int64_t beg = A.pin.seg_beg[s];
-    const int64_t end = beg + A.pin.seg_len[s];
-    uint64_t gj = ms_seg_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], use_fb, scanned);
-    if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
+    if (tally_on) tally_words(got, clamp_deg(od), clamp_deg(indeg), sym, tally);
   }
   if (tally_on) tally_flush(tally, A.lanes, tl);
   live_flush(live, &c->live);
@@ -1077,61 +1104,38 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
 }
 
-// Big vertices after a pull round: turn per-lane candidates into outputs.
-__global__ void __launch_bounds__(kMsBlock) ms_pull_fin(MsArgs A, int tally_on) {
+// Big vertices of a pull round, before ms_pull: one active segment at a time
+// per wavefront (ms_active_segs listed the segments whose vertex needs lanes,
+// with those lanes).  First hits go to the vertex's per-lane candidates by
+// atomicMin, the lanes found to bgot; ms_pull writes them out with the
+// vertex's chunk.  Segments come from their own cursors, kSegGrab a grab (a
+// segment stops at its first step that finds every needed lane, so costs
+// differ), or by a static split when there are few.
+constexpr int32_t kSegGrab = 4;
+__global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
   if (!R.pm) return;
-  const int d = R.d;
+  const int32_t nact = (int32_t)ld_ctrl(&c->segtail[0]);
+  if (nact == 0) return;  // uniform: written by the previous kernel
   __shared__ unsigned long long red[kMsWaves];
-  __shared__ LaneOutLds so;
-  stage_out(A.out, d, so);
-  __shared__ unsigned long long tl[192];
-  tally_lds_init(tl);
-  LaneTally tally;
-  uint64_t *nx = A.fw[(d + 1) & 1];
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long nf = 0, pairs = 0, mf = 0;
-  uint64_t live = 0;
-  // wavefront-uniform trip count (the tally is a wavefront operation)
-  for (int64_t b0 = gtid - lane_id(); b0 < A.pin.nbig; b0 += stride) {
-    const int64_t b = b0 + lane_id();
-    uint64_t got = 0;
-    int64_t od = 0, id = 0;
-    if (b < A.pin.nbig) {
-      got = A.bgot[b];
-      if (got) {
-        const int32_t v = A.pin.big_v[b];
-        uint64_t lanes = got;
-        while (lanes) {
-          int l = __ffsll((unsigned long long)lanes) - 1;
-          lanes &= lanes - 1;
-          st_once(&so.par[l][v], (int32_t)A.cand[b * 64 + l]);
-          st_once(&so.lev[l][v], so.level[l]);
-          A.cand[b * 64 + l] = 0xFFFFFFFFu;
-        }
-        A.bgot[b] = 0;
-        nx[A.slot_of[v]] = got;
-        A.vis[v] |= got;
-        id = A.in_ptr[v + 1] - A.in_ptr[v];
-        od = A.symmetric ? id : A.out_ptr[v + 1] - A.out_ptr[v];
-        mf += (unsigned long long)od;
-        live |= got;
-        pairs += (unsigned long long)__popcll(got);
-        nf++;
-      }
-    }
-    if (tally_on) tally_words(got, clamp_deg(od), clamp_deg(id), A.symmetric != 0, tally);
+  const uint64_t *fr = A.fw[R.d & 1];
+  const int lane = lane_id();
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  uint32_t scanned = 0;
+  ChunkFeed feed(A.cursor + kParts * kCursorStride, nact, kSegGrab, nact < kSegGrab * nwaves);
+  int32_t gb, ge;
+  while (feed.next(gb, ge)) for (int32_t i = gb; i < ge; i++) {
+    const int32_t s = A.segq[0][i];
+    const uint64_t need = A.segneed[i];
+    const int32_t bid = A.pin.seg_bid[s];
+    const int64_t beg = A.pin.seg_beg[s];
+    const int64_t end = beg + A.pin.seg_len[s];
+    uint64_t gj = ms_seg_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], R.use_fb, scanned);
+    if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
   }
-  if (tally_on) tally_flush(tally, A.lanes, tl);
-  live_flush(live, &c->live);
-  unsigned long long s = ms_block_sum(nf, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->pnf, s);
-  s = ms_block_sum(pairs, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->ppairs, s);
-  s = ms_block_sum(mf, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  const unsigned long long t = ms_block_sum(scanned, red);
+  if (threadIdx.x == 0 && t) atomicAdd(&c->pscanned, t);
 }
 
 // Active list from the frontier words (slot order -> vertex ids), for a push
@@ -1477,7 +1481,7 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   st_ctrl(&c->live, 0ull);
   st_ctrl(&c->segtail[0], 0ull);
   st_ctrl(&c->segtail[1], 0ull);
-  reset_cursors(A.cursor);
+  reset_ms_cursors(A.cursor);
   if (cur_pm) st_ctrl(&c->noin_left, 0);  // this round's pull visited every vertex
   // A pull rescans every row it opens from the start, so the next pull reads
   // at most what this one read: below one entry per vertex it is light, and
@@ -1672,7 +1676,7 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&c->status, 0);
     st_ctrl(&c->grab, 1);
     st_ctrl(&c->noin_left, 0);  // retiring lanes write their own
-    reset_cursors(A.cursor);
+    reset_ms_cursors(A.cursor);
   }
 }
 
@@ -1915,9 +1919,9 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
   };
   auto pull_kernels = [&]() {
+    if (md->pin.nbig) ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A);
     if (tally) ms_pull<true><<<pblocks, kMsBlock, 0, st>>>(A);
     else ms_pull<false><<<pblocks, kMsBlock, 0, st>>>(A);
-    if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, tally);
   };
   auto push_part = [&](bool compact) {
     if (compact) ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
@@ -2161,9 +2165,9 @@ static void eng_round(cgbfs_engine *E) {
   };
   ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
   if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
+  if (md->pin.nbig) ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A);
   if (E->tally) ms_pull<true><<<E->pblocks, kMsBlock, 0, st>>>(A);
   else ms_pull<false><<<E->pblocks, kMsBlock, 0, st>>>(A);
-  if (md->pin.nbig) ms_pull_fin<<<(unsigned)((md->pin.nbig + kMsBlock - 1) / kMsBlock), kMsBlock, 0, st>>>(A, E->tally);
   ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
   if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
   ms_push<<<blocks, kMsBlock, 0, st>>>(A, 0);
