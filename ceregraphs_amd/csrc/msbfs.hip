@@ -1108,10 +1108,14 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
 // per wavefront (ms_active_segs listed the segments whose vertex needs lanes,
 // with those lanes).  First hits go to the vertex's per-lane candidates by
 // atomicMin, the lanes found to bgot; ms_pull writes them out with the
-// vertex's chunk.  Segments come from their own cursors, kSegGrab a grab (a
-// segment stops at its first step that finds every needed lane, so costs
-// differ), or by a static split when there are few.
-constexpr int32_t kSegGrab = 4;
+// vertex's chunk.  Segments come from their own cursors, kSegGrab
+// consecutive ones a grab: a segment stops at its first step that finds
+// every needed lane, so costs differ, and a long run of one vertex's
+// segments in row order lets the later ones skip the lanes the earlier ones
+// found.  With fewer than kSegStatic segments per wavefront they are split
+// statically instead.
+constexpr int32_t kSegGrab = 32;
+constexpr int32_t kSegStatic = 4;
 __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
@@ -1123,15 +1127,26 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A) {
   const int lane = lane_id();
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
   uint32_t scanned = 0;
-  ChunkFeed feed(A.cursor + kParts * kCursorStride, nact, kSegGrab, nact < kSegGrab * nwaves);
+  ChunkFeed feed(A.cursor + kParts * kCursorStride, nact, kSegGrab, nact < kSegStatic * nwaves);
   int32_t gb, ge;
+  // A vertex's segments are listed together in row order, so lanes this
+  // wavefront found in an earlier segment (lower ids) are not looked for in
+  // a later one of the same vertex.
+  int32_t prev_bid = -1, prev_i = -1;
+  uint64_t prev_got = 0;
   while (feed.next(gb, ge)) for (int32_t i = gb; i < ge; i++) {
     const int32_t s = A.segq[0][i];
-    const uint64_t need = A.segneed[i];
+    uint64_t need = A.segneed[i];
     const int32_t bid = A.pin.seg_bid[s];
+    if (bid == prev_bid && i > prev_i) need &= ~prev_got;
+    else prev_got = 0;
+    prev_bid = bid;
+    prev_i = i;
+    if (!need) continue;
     const int64_t beg = A.pin.seg_beg[s];
     const int64_t end = beg + A.pin.seg_len[s];
     uint64_t gj = ms_seg_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], R.use_fb, scanned);
+    prev_got |= gj;
     if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
   }
   const unsigned long long t = ms_block_sum(scanned, red);
