@@ -77,6 +77,7 @@ struct MsCtrl {
   unsigned long long mf;       // out-degree sum of the vertices with a new lane
   unsigned long long live;     // lanes that reached a vertex this round
   unsigned long long segtail[2];  // active segments this round: pull, push
+  unsigned long long segtail_late;  // active pull segments past their vertex's first run
   unsigned long long pmask;    // lanes the next round pulls
   unsigned long long qmask;    // lanes the next round pushes
   long long prev_nf;
@@ -176,8 +177,12 @@ struct MsArgs {
   SegList pin, pout;
   int32_t *segq[2];
   uint64_t *segneed;  // [pin.nseg] lanes needed by each active pull segment
-  // [2 * kParts * kCursorStride] pull cursors, zero between rounds: chunks,
-  // then active big-vertex segments
+  // The active pull segments past the first kSegRun of their vertex, and
+  // their lanes (pulled after every first run has finished).
+  int32_t *segq_late;
+  uint64_t *segneed_late;
+  // [3 * kParts * kCursorStride] pull cursors, zero between rounds: chunks,
+  // first-run segments, later segments
   unsigned long long *cursor;
   const uint64_t *noin;  // vertices without in-edges, one bit each
   int *host_done;        // pinned host flag raised with done (graph replay), or null
@@ -230,6 +235,8 @@ struct MsWorkspace {
   uint64_t *bgot;
   int32_t *segq[2];
   uint64_t *segneed;
+  int32_t *segq_late;
+  uint64_t *segneed_late;
   unsigned long long *cursor;
 };
 
@@ -406,10 +413,10 @@ static void free_msdata(void *q) {
   delete m;
 }
 
-// Both sets of pull cursors (chunks, then big-vertex segments) back to zero.
+// The three sets of pull cursors (chunks, first-run segments, later
+// segments) back to zero.
 __device__ __forceinline__ void reset_ms_cursors(unsigned long long *cur) {
-  reset_cursors(cur);
-  reset_cursors(cur + kParts * kCursorStride);
+  for (int k = 0; k < 3; k++) reset_cursors(cur + k * kParts * kCursorStride);
 }
 
 // Lays a multi-source workspace out from `base` (sizes only when base is
@@ -423,7 +430,8 @@ static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspac
       al(sizeof(LaneOut)), al(sizeof(unsigned long long) * 64),
       al(sizeof(uint32_t) * 64 * (nbig + 1)), al(sizeof(uint64_t) * (nbig + 1)),
       al(sizeof(int32_t) * (md->pin.nseg + 1)), al(sizeof(int32_t) * (md->pout.nseg + 1)),
-      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(unsigned long long) * 2 * kParts * kCursorStride)};
+      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(int32_t) * (md->pin.nseg + 1)),
+      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(unsigned long long) * 3 * kParts * kCursorStride)};
   size_t total = 0;
   for (size_t x : sizes) total += x;
   if (!base) return total;
@@ -446,6 +454,8 @@ static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspac
   w->segq[0] = (int32_t *)take();
   w->segq[1] = (int32_t *)take();
   w->segneed = (uint64_t *)take();
+  w->segq_late = (int32_t *)take();
+  w->segneed_late = (uint64_t *)take();
   w->cursor = (unsigned long long *)take();
   return total;
 }
@@ -852,6 +862,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->live, 0ull);
     st_ctrl(&c->segtail[0], 0ull);
     st_ctrl(&c->segtail[1], 0ull);
+    st_ctrl(&c->segtail_late, 0ull);
     const long long nq0 = (long long)ld_ctrl(&c->qtail[0]);
     st_ctrl(&c->prev_nf, nq0);
     st_ctrl(&c->listed, 1);
@@ -1108,26 +1119,34 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
 // per wavefront (ms_active_segs listed the segments whose vertex needs lanes,
 // with those lanes).  First hits go to the vertex's per-lane candidates by
 // atomicMin, the lanes found to bgot; ms_pull writes them out with the
-// vertex's chunk.  Segments come from their own cursors, kSegGrab
-// consecutive ones a grab: a segment stops at its first step that finds
-// every needed lane, so costs differ, and a long run of one vertex's
+// vertex's chunk.  Two launches: late = 0 pulls the first kSegRun segments
+// of every vertex, late = 1 the rest, where a lane whose candidate (from
+// the finished first launch, or from this one) is already below the
+// segment's first id cannot improve and drops out; a stale read of a
+// candidate only keeps a lane in.  Segments come from their own cursors,
+// kSegGrab consecutive ones a grab: a segment stops at its first step that
+// finds every needed lane, so costs differ, and a run of one vertex's
 // segments in row order lets the later ones skip the lanes the earlier ones
 // found.  With fewer than kSegStatic segments per wavefront they are split
 // statically instead.
 constexpr int32_t kSegGrab = 32;
+constexpr int32_t kSegRun = 32;
 constexpr int32_t kSegStatic = 4;
-__global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A) {
+__global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A, int late) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
   if (!R.pm) return;
-  const int32_t nact = (int32_t)ld_ctrl(&c->segtail[0]);
-  if (nact == 0) return;  // uniform: written by the previous kernel
+  const int32_t nact = (int32_t)ld_ctrl(late ? &c->segtail_late : &c->segtail[0]);
+  if (nact == 0) return;  // uniform: written by an earlier kernel
   __shared__ unsigned long long red[kMsWaves];
   const uint64_t *fr = A.fw[R.d & 1];
   const int lane = lane_id();
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+  const int32_t *segs = late ? A.segq_late : A.segq[0];
+  const uint64_t *needs = late ? A.segneed_late : A.segneed;
   uint32_t scanned = 0;
-  ChunkFeed feed(A.cursor + kParts * kCursorStride, nact, kSegGrab, nact < kSegStatic * nwaves);
+  ChunkFeed feed(A.cursor + (1 + late) * kParts * kCursorStride, nact, kSegGrab,
+                 nact < kSegStatic * nwaves);
   int32_t gb, ge;
   // A vertex's segments are listed together in row order, so lanes this
   // wavefront found in an earlier segment (lower ids) are not looked for in
@@ -1135,15 +1154,19 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A) {
   int32_t prev_bid = -1, prev_i = -1;
   uint64_t prev_got = 0;
   while (feed.next(gb, ge)) for (int32_t i = gb; i < ge; i++) {
-    const int32_t s = A.segq[0][i];
-    uint64_t need = A.segneed[i];
+    const int32_t s = segs[i];
+    uint64_t need = needs[i];
     const int32_t bid = A.pin.seg_bid[s];
     if (bid == prev_bid && i > prev_i) need &= ~prev_got;
     else prev_got = 0;
     prev_bid = bid;
     prev_i = i;
-    if (!need) continue;
     const int64_t beg = A.pin.seg_beg[s];
+    if (late && need) {
+      const uint32_t low = A.cand[(int64_t)bid * 64 + lane];
+      need &= ~ballot64(low < (uint32_t)A.in_idx[beg]);
+    }
+    if (!need) continue;
     const int64_t end = beg + A.pin.seg_len[s];
     uint64_t gj = ms_seg_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], R.use_fb, scanned);
     prev_got |= gj;
@@ -1203,20 +1226,32 @@ __global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int which) 
         cnt = L.seg_first[b + 1] - first;
       }
     }
-    int32_t incl = cnt;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      int32_t y = __shfl_up(incl, off);
-      if (lane >= off) incl += y;
-    }
+    // A pull lists a vertex's first kSegRun segments in segq, the rest in
+    // segq_late; both keep row order.
+    const int32_t c0 = which ? cnt : (cnt < kSegRun ? cnt : kSegRun);
+    const int32_t c1 = cnt - c0;
+    const int32_t incl = wave_scan_add(c0);
     const int32_t total = __shfl(incl, kWave - 1);
-    if (total == 0) continue;
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(tail, (unsigned long long)total);
-    base = __shfl(base, 0) + (unsigned long long)(incl - cnt);
-    for (int32_t k = 0; k < cnt; k++) out[base + k] = first + k;
-    if (!which)
-      for (int32_t k = 0; k < cnt; k++) A.segneed[base + k] = need;
+    if (total) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(tail, (unsigned long long)total);
+      base = __shfl(base, 0) + (unsigned long long)(incl - c0);
+      for (int32_t k = 0; k < c0; k++) out[base + k] = first + k;
+      if (!which)
+        for (int32_t k = 0; k < c0; k++) A.segneed[base + k] = need;
+    }
+    if (which) continue;
+    const int32_t incl1 = wave_scan_add(c1);
+    const int32_t total1 = __shfl(incl1, kWave - 1);
+    if (total1) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&A.ctrl->segtail_late, (unsigned long long)total1);
+      base = __shfl(base, 0) + (unsigned long long)(incl1 - c1);
+      for (int32_t k = 0; k < c1; k++) {
+        A.segq_late[base + k] = first + c0 + k;
+        A.segneed_late[base + k] = need;
+      }
+    }
   }
 }
 
@@ -1496,6 +1531,7 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   st_ctrl(&c->live, 0ull);
   st_ctrl(&c->segtail[0], 0ull);
   st_ctrl(&c->segtail[1], 0ull);
+  st_ctrl(&c->segtail_late, 0ull);
   reset_ms_cursors(A.cursor);
   if (cur_pm) st_ctrl(&c->noin_left, 0);  // this round's pull visited every vertex
   // A pull rescans every row it opens from the start, so the next pull reads
@@ -1680,6 +1716,7 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&c->live, 0ull);
     st_ctrl(&c->segtail[0], 0ull);
     st_ctrl(&c->segtail[1], 0ull);
+    st_ctrl(&c->segtail_late, 0ull);
     st_ctrl(&c->pmask, 0ull);
     st_ctrl(&c->qmask, 0ull);
     st_ctrl(&c->prev_nf, 0ll);
@@ -1877,6 +1914,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   A.segq[0] = ws.segq[0];
   A.segq[1] = ws.segq[1];
   A.segneed = ws.segneed;
+  A.segq_late = ws.segq_late;
+  A.segneed_late = ws.segneed_late;
   A.cursor = ws.cursor;
   A.noin = md->noin;
   A.host_done = nullptr;
@@ -1934,7 +1973,10 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
   };
   auto pull_kernels = [&]() {
-    if (md->pin.nbig) ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A);
+    if (md->pin.nbig) {
+      ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 0);
+      ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 1);
+    }
     if (tally) ms_pull<true><<<pblocks, kMsBlock, 0, st>>>(A);
     else ms_pull<false><<<pblocks, kMsBlock, 0, st>>>(A);
   };
@@ -2180,7 +2222,10 @@ static void eng_round(cgbfs_engine *E) {
   };
   ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
   if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
-  if (md->pin.nbig) ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A);
+  if (md->pin.nbig) {
+    ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 0);
+    ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 1);
+  }
   if (E->tally) ms_pull<true><<<E->pblocks, kMsBlock, 0, st>>>(A);
   else ms_pull<false><<<E->pblocks, kMsBlock, 0, st>>>(A);
   ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
@@ -2245,6 +2290,8 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   A.segq[0] = E->ws.segq[0];
   A.segq[1] = E->ws.segq[1];
   A.segneed = E->ws.segneed;
+  A.segq_late = E->ws.segq_late;
+  A.segneed_late = E->ws.segneed_late;
   A.cursor = E->ws.cursor;
   A.noin = E->md->noin;
   A.host_done = nullptr;
