@@ -214,6 +214,37 @@ def test_msbfs_edge_cases():
       _check_ms_stats(r.stats, levs, d)
 
 
+def test_msbfs_hub_segments_lowest_id():
+  """A hub with 20000 in-neighbours: 79 segments of 256 entries, the first
+  32 in one list and 47 in the late one.  Lane l's root links to one to four
+  of the hub's in-neighbours: lanes 0-15 only below id 8192 (the first 32
+  segments), lanes 16-31 only above, lanes 32-63 anywhere, so the hub's
+  parent in a pull round (the lowest of them) comes from either launch, and
+  lanes found early must be dropped, not lost, in the later segments."""
+  hub_deg = 20000
+  hub = hub_deg + 64
+  rng = np.random.default_rng(11)
+  e = [(j, hub) for j in range(hub_deg)]
+  roots = np.arange(hub_deg, hub_deg + 64)
+  for l in range(64):
+    k = 1 + l % 4
+    lo, hi = (0, 8192) if l < 16 else (8192, hub_deg) if l < 32 else (0, hub_deg)
+    e += [(int(roots[l]), int(p)) for p in rng.integers(lo, hi, size=k)]
+  g = _edges_graph(e, hub + 1, directed=False)
+  assert g.in_ptr[hub + 1] - g.in_ptr[hub] == hub_deg
+  levs = np.stack([O.bfs(g, int(s))[0] for s in roots])
+  assert np.all(levs[:, hub] == 2)
+  with _device(g) as dg:
+    for d, cap in [(d, False) for d in DIRECTIONS + ["beamer_lanes"]] + [("bottomup", True)]:
+      r = dg.msbfs(roots, direction=d, capture=cap)
+      for l, s in enumerate(roots):
+        lev, par, _, _ = O.bfs(g, int(s))
+        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16),
+                                      err_msg=f"lane {l} dir {d} capture {cap}")
+        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d} capture {cap}")
+      _check_ms_stats(r.stats, levs, d)
+
+
 def _check_ms_stats(st, levs, direction):
   """Round accounting of a multi-source pass against the lanes' oracle
   levels: round r reaches the (lane, vertex) pairs at level r+1, nf counts
