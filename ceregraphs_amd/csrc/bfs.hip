@@ -33,7 +33,7 @@ constexpr int kHistCap = 4096;        // rounds recorded in the history arrays
 constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/above this
 constexpr int kBigLds = 4096;         // push: big-list prefix entries staged per pass
 constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this
-constexpr int64_t kPullWaveDegree = 128;  // pull: wavefront scan at/above this
+constexpr int64_t kPullWaveDegree = 256;  // pull: wavefront scan at/above this
 constexpr int kPullUnroll = 4;         // pull: row entries a lane tests per step
 constexpr int32_t kPullGrab = 4;       // pull: chunks per grab from the cursors
 constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
