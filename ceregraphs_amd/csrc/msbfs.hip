@@ -17,13 +17,14 @@
 //                       item, and segments of one vertex combine per lane
 //                       with atomicMin / atomicOr in a small scratch area.
 //
-// Directions are chosen per lane: each lane runs the single-root rule on its
-// own counts (its frontier size and degree sums, tallied per lane bit in
-// the kernels), and a round first pulls the lanes in pull mode, then pushes
-// the lanes in push mode.  Roots progress at different speeds (a root next
-// to a hub is a level ahead of a leaf root), so one direction for all lanes
-// would make the pull rescan whole adjacency lists for lanes that are not
-// there yet.  Lanes with an empty frontier take no part.
+// Directions: by default (CGBFS_DIR_BEAMER) every lane moves together, by
+// Beamer's rule on the union counts of the round.  The per-lane policies
+// (CGBFS_DIR_BEAMER_LANES, CGBFS_DIR_TAU) run the single-root rule on each
+// lane's own counts (its frontier size and degree sums, tallied per lane
+// bit in the kernels), and such a round first pulls the lanes in pull mode,
+// then pushes the lanes in push mode.  On R-MAT graphs the mixed rounds
+// cost more than the rescans they save, so the union rule is the default.
+// Lanes with an empty frontier take no part.
 // Pull part (ms_pull_segs + ms_pull): every vertex with an unreached pull
 // lane ORs the frontier words of its ascending in-neighbours; within a step
 // of 64 neighbours an exclusive prefix-OR across the wavefront gives each
