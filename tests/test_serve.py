@@ -33,27 +33,18 @@ def test_latency_summary():
   assert 4.0 < s["p99_ms"] <= 100.0
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("scale,ef,seed,directed,direction", [
-    (13, 16, 21, False, "beamer_lanes"), (12, 8, 22, True, "beamer_lanes"),
-    (12, 16, 23, False, "topdown"), (11, 16, 24, False, "bottomup"), (12, 8, 25, True, "tau")])
-def test_engine_results_equal_alone_and_in_mix(scale, ef, seed, directed, direction):
+def _serve_staggered(dg, g, roots, direction):
+  """Runs the queries through one engine, up to 7 joining per step while
+  lanes are free, and checks every result, m and round count against the
+  oracle; returns the result tensors."""
   import torch
-  abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
-  n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
-  g = O.Graph(n, ip, ii, op, oi, not directed)
-  rng = np.random.default_rng(seed)
-  nq = 150  # more queries than lanes: lanes are reused
-  roots = rng.integers(0, n, size=nq)
-  roots[:3] = [int(np.argmax(np.diff(op))), roots[5], roots[5]]  # a hub, a repeated root
+  n, nq = g.n, len(roots)
   par = [torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(nq)]
   lev = [torch.empty(n, dtype=torch.int16, device="cuda") for _ in range(nq)]
-  with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg, \
-       ServingEngine(dg, direction=direction) as eng:
+  with ServingEngine(dg, direction=direction) as eng:
     free, lane_of, nxt, done = list(range(64)), {}, 0, {}
     step = 0
     while len(done) < nq:
-      # staggered arrivals: up to 7 new queries per step while lanes are free
       for _ in range(7):
         if nxt < nq and free:
           lane = free.pop(0)
@@ -69,12 +60,54 @@ def test_engine_results_equal_alone_and_in_mix(scale, ef, seed, directed, direct
         free.append(lane)
       step += 1
       assert step < 100000
-    for q in range(nq):
-      lev0, par0, _, rounds = O.bfs(g, int(roots[q]))
-      np.testing.assert_array_equal(lev[q].cpu().numpy(), lev0.astype(np.int16), err_msg=f"query {q}")
-      np.testing.assert_array_equal(par[q].cpu().numpy(), par0, err_msg=f"query {q}")
-      assert done[q] == (O.m_edges(g, lev0), rounds), f"query {q}"
+  for q in range(nq):
+    lev0, par0, _, rounds = O.bfs(g, int(roots[q]))
+    np.testing.assert_array_equal(lev[q].cpu().numpy(), lev0.astype(np.int16), err_msg=f"query {q}")
+    np.testing.assert_array_equal(par[q].cpu().numpy(), par0, err_msg=f"query {q}")
+    assert done[q] == (O.m_edges(g, lev0), rounds), f"query {q}"
+  return par, lev
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scale,ef,seed,directed,direction", [
+    (13, 16, 21, False, "beamer_lanes"), (12, 8, 22, True, "beamer_lanes"),
+    (12, 16, 23, False, "topdown"), (11, 16, 24, False, "bottomup"), (12, 8, 25, True, "tau")])
+def test_engine_results_equal_alone_and_in_mix(scale, ef, seed, directed, direction):
+  abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+  g = O.Graph(n, ip, ii, op, oi, not directed)
+  rng = np.random.default_rng(seed)
+  nq = 150  # more queries than lanes: lanes are reused
+  roots = rng.integers(0, n, size=nq)
+  roots[:3] = [int(np.argmax(np.diff(op))), roots[5], roots[5]]  # a hub, a repeated root
+  with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
+    par, lev = _serve_staggered(dg, g, roots, direction)
     for q in range(0, nq, 15):  # the same query alone, single-root kernel
       alone = dg.bfs(int(roots[q]))
       np.testing.assert_array_equal(lev[q].cpu().numpy(), alone.levels.astype(np.int16))
       np.testing.assert_array_equal(par[q].cpu().numpy(), alone.parents)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("direction", ["bottomup", "beamer_lanes"])
+def test_engine_hub_with_late_segments(direction):
+  """A hub with 20000 in-neighbours (32 segments in the first list, 47 in
+  the late one) pulled by queries that join at different rounds: 64 roots
+  each linked to one to four of its in-neighbours (below id 8192 only,
+  above only, or anywhere), then 40 of those in-neighbours as roots."""
+  import scipy.sparse as sp
+  hub_deg = 20000
+  hub = hub_deg + 64
+  rng = np.random.default_rng(11)
+  src, dst = [j for j in range(hub_deg)], [hub] * hub_deg
+  for l in range(64):
+    lo, hi = (0, 8192) if l < 16 else (8192, hub_deg) if l < 32 else (0, hub_deg)
+    for p in rng.integers(lo, hi, size=1 + l % 4):
+      src.append(hub_deg + l)
+      dst.append(int(p))
+  src, dst = np.array(src + dst), np.array(dst + src)  # undirected
+  M = sp.coo_matrix((np.ones(len(src)), (dst, src)), shape=(hub + 1, hub + 1))
+  g = O.Graph.from_matrix(M)
+  roots = np.concatenate([np.arange(hub_deg, hub_deg + 64), rng.integers(0, hub_deg, size=40)])
+  with DeviceGraph.from_csr(g.n, g.out_ptr, g.out_idx, g.in_ptr, g.in_idx, g.symmetric) as dg:
+    _serve_staggered(dg, g, roots, direction)
