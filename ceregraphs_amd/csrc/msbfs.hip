@@ -56,7 +56,7 @@ constexpr int kMsHist = 4096;
 constexpr int64_t kSeg = 256;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
-constexpr int kFlush = 8;           // lane bits per batch of the pull's output flush
+constexpr int kFlush = 16;          // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
 constexpr int32_t kGrabSparse = 8;  // marks a pull known to be light: it splits its chunks statically
 // First-hit offsets within a wavefront-tier row (< kSeg) fit in a byte.
