@@ -28,6 +28,7 @@ DIRECTIONS = {"beamer": DIR_BEAMER, "topdown": DIR_TOPDOWN, "bottomup": DIR_BOTT
 # Every symbol include/cgbfs.h declares.
 EXPORTED = (
     "cgbfs_last_error", "cgbfs_version", "cgbfs_device_count", "cgbfs_graph_create",
+    "cgbfs_graph_check",
     "cgbfs_graph_create_rmat", "cgbfs_graph_info", "cgbfs_graph_download",
     "cgbfs_graph_out_degree", "cgbfs_graph_destroy", "cgbfs_bfs", "cgbfs_msbfs",
     "cgbfs_msbfs_seeded", "cgbfs_sync", "cgbfs_engine_create", "cgbfs_engine_submit",
@@ -90,6 +91,8 @@ _SIGNATURES = {
     "cgbfs_device_count": (ctypes.c_int, [_I32P]),
     "cgbfs_graph_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _P, _P,
                                           _P, _P, ctypes.c_int32, ctypes.POINTER(_GP)]),
+    "cgbfs_graph_check": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P,
+                                         ctypes.c_int32]),
     "cgbfs_graph_create_rmat": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
                                                _U64P, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_int32,

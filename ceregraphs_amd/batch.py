@@ -126,10 +126,12 @@ class UnevenBatch:
 
   @classmethod
   def from_rmat(cls, specs, abcd=DIRECTED_ABCD, directed=True, device=0, keep_parts=False,
-                balance_grid=UNEVEN_BALANCE_GRID):
+                balance_grid=UNEVEN_BALANCE_GRID, keep_host=False):
     """Generates the graphs on the GPU, balances each on a balance_grid x
     balance_grid grid (0: keep the generator's ids) and packs the union.
-    keep_parts also keeps every graph as a device graph of its own."""
+    keep_parts also keeps every graph as a device graph of its own;
+    keep_host keeps the (balanced) host arrays of every graph as
+    batch.host_parts, (n, in_ptr, in_idx, out_ptr, out_idx) each."""
     parts, graphs, reports = [], [], []
     for scale, ef, seed in specs:
       g = DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed, device=device)
@@ -149,6 +151,7 @@ class UnevenBatch:
         g.close()
     batch = cls(parts, symmetric=not directed, device=device, graphs=graphs)
     batch.balance = reports
+    batch.host_parts = parts if keep_host else None
     return batch
 
   def close(self):

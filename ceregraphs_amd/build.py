@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcgbfs.so")
-SOURCES = ["graph.hip", "bfs.hip", "msbfs.hip"]
+SOURCES = ["graph.hip", "check.hip", "bfs.hip", "msbfs.hip"]
 ARCH = os.environ.get("CGBFS_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -70,6 +70,14 @@ struct DeviceGraph {
 
 int graph_finalize(DeviceGraph *g);  // computes m_weight, stream, mailbox
 
+// Structural checks of a graph (check.hip): offsets 0..nnz non-decreasing,
+// ids in [0, n), rows strictly ascending, in view == transpose of out view.
+// The host check reads host arrays; the device check reads g's own arrays
+// on g's stream.  Both return CGBFS_OK or CGBFS_EINVAL (message set).
+int check_graph_host(int64_t n, int64_t nnz, const int64_t *out_ptr, const int32_t *out_idx,
+                     const int64_t *in_ptr, const int32_t *in_idx, int symmetric);
+int check_graph_device(const DeviceGraph *g);
+
 // HIP events owned for the length of one call (destroyed on every exit path).
 struct EventPool {
   std::vector<hipEvent_t> ev;

@@ -301,6 +301,20 @@ static void graph_free(DeviceGraph *g) {
   delete reinterpret_cast<cgbfs_graph *>(g);
 }
 
+// 1: device memory of `device`; -1: device memory of another device; 0: host
+// memory (pageable, pinned or managed).  A pointer the runtime does not know
+// is host memory.
+static int pointer_kind(const void *p, int device) {
+  hipPointerAttribute_t a;
+  std::memset(&a, 0, sizeof(a));
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (a.type != hipMemoryTypeDevice || a.isManaged) return 0;
+  return a.device == device ? 1 : -1;
+}
+
 static int check_device(int32_t device) {
   int count = 0;
   hipError_t e = hipGetDeviceCount(&count);
@@ -337,10 +351,25 @@ int cgbfs_graph_create(int32_t device, int64_t n, int64_t nnz, const int64_t *ou
   if (!out_ptr || (nnz > 0 && !out_idx)) return set_error(CGBFS_EINVAL, "out_ptr/out_idx null");
   if (!symmetric && (!in_ptr || (nnz > 0 && !in_idx)))
     return set_error(CGBFS_EINVAL, "directed graph needs in_ptr/in_idx");
-  if (out_ptr[0] != 0 || out_ptr[n] != nnz) return set_error(CGBFS_EINVAL, "out_ptr must run 0..nnz");
-  if (!symmetric && (in_ptr[0] != 0 || in_ptr[n] != nnz))
-    return set_error(CGBFS_EINVAL, "in_ptr must run 0..nnz");
-  int rc = check_device(device);
+  // Host or device arrays: all of one kind.
+  const void *arrs[4] = {out_ptr, nnz > 0 ? out_idx : nullptr, symmetric ? nullptr : in_ptr,
+                         symmetric || nnz == 0 ? nullptr : in_idx};
+  int kinds = 0;  // bit 0: host array seen, bit 1: device array of `device`
+  for (const void *a : arrs) {
+    if (!a) continue;
+    const int k = pointer_kind(a, device);
+    if (k < 0) return set_error(CGBFS_EINVAL, "graph arrays live on another device");
+    kinds |= k ? 2 : 1;
+  }
+  if (kinds == 3) return set_error(CGBFS_EINVAL, "graph arrays mix host and device memory");
+  const bool on_dev = kinds == 2;
+  int rc = CGBFS_OK;
+  if (!on_dev) {
+    // host arrays: checked before anything touches the GPU
+    rc = check_graph_host(n, nnz, out_ptr, out_idx, in_ptr, in_idx, symmetric ? 1 : 0);
+    if (rc) return rc;
+  }
+  rc = check_device(device);
   if (rc) return rc;
   cgbfs_graph *h = new cgbfs_graph();
   DeviceGraph *g = &h->g;
@@ -348,12 +377,14 @@ int cgbfs_graph_create(int32_t device, int64_t n, int64_t nnz, const int64_t *ou
   g->n = n;
   g->nnz = nnz;
   g->symmetric = symmetric ? 1 : 0;
+  const hipMemcpyKind kind = on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   size_t pbytes = sizeof(int64_t) * (size_t)(n + 1);
   size_t ibytes = sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1);
-  hipError_t e = hipMalloc(&g->out_ptr, pbytes);
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&g->out_ptr, pbytes);
   if (e == hipSuccess) e = hipMalloc(&g->out_idx, ibytes);
-  if (e == hipSuccess) e = hipMemcpy(g->out_ptr, out_ptr, pbytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess && nnz > 0) e = hipMemcpy(g->out_idx, out_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->out_ptr, out_ptr, pbytes, kind);
+  if (e == hipSuccess && nnz > 0) e = hipMemcpy(g->out_idx, out_idx, sizeof(int32_t) * nnz, kind);
   if (e == hipSuccess) {
     if (symmetric) {
       g->in_ptr = g->out_ptr;
@@ -361,8 +392,8 @@ int cgbfs_graph_create(int32_t device, int64_t n, int64_t nnz, const int64_t *ou
     } else {
       e = hipMalloc(&g->in_ptr, pbytes);
       if (e == hipSuccess) e = hipMalloc(&g->in_idx, ibytes);
-      if (e == hipSuccess) e = hipMemcpy(g->in_ptr, in_ptr, pbytes, hipMemcpyHostToDevice);
-      if (e == hipSuccess && nnz > 0) e = hipMemcpy(g->in_idx, in_idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(g->in_ptr, in_ptr, pbytes, kind);
+      if (e == hipSuccess && nnz > 0) e = hipMemcpy(g->in_idx, in_idx, sizeof(int32_t) * nnz, kind);
     }
   }
   if (e != hipSuccess) {
@@ -370,7 +401,10 @@ int cgbfs_graph_create(int32_t device, int64_t n, int64_t nnz, const int64_t *ou
     graph_free(g);
     return rc;
   }
-  rc = graph_finalize(g);
+  // device arrays: checked on the library's own copy, before any kernel
+  // that trusts the structure (m_weight below included) reads it
+  if (on_dev) rc = check_graph_device(g);
+  if (!rc) rc = graph_finalize(g);
   if (rc) {
     graph_free(g);
     return rc;

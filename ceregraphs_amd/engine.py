@@ -131,20 +131,37 @@ class DeviceGraph:
   # -- construction --------------------------------------------------------
   @classmethod
   def from_csr(cls, n, out_ptr, out_idx, in_ptr=None, in_idx=None, symmetric=False, device=0):
-    """Uploads host arrays.  out_* = column-compressed M (push view),
-    in_* = row-compressed M (pull view, ascending); symmetric graphs may
-    pass only the out view."""
-    out_ptr = np.ascontiguousarray(out_ptr, dtype=np.int64)
-    out_idx = np.ascontiguousarray(out_idx, dtype=np.int32)
+    """Copies a graph into the library.  out_* = column-compressed M (push
+    view), in_* = row-compressed M (pull view, ascending); symmetric graphs
+    may pass only the out view.  The arrays are host arrays (numpy, int64
+    offsets and int32 ids) or CUDA tensors of those dtypes on `device`, all
+    of one kind; device tensors are copied device to device and checked on
+    the GPU."""
     if symmetric:
       in_ptr = in_idx = None
+    arrays = [out_ptr, out_idx, in_ptr, in_idx]
+    on_dev = [a is not None and hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+              for a in arrays]
+    if any(on_dev):
+      keep = arrays
+      for a, want in zip(arrays, ("int64", "int32", "int64", "int32")):
+        if a is not None:
+          _check_tensor(a, a.numel(), want)
+      _after_torch_stream(out_ptr)
+      ptrs = [ctypes.c_void_p(a.data_ptr()) if a is not None else None for a in arrays]
+      nnz = int(out_idx.numel())
     else:
-      in_ptr = np.ascontiguousarray(in_ptr, dtype=np.int64)
-      in_idx = np.ascontiguousarray(in_idx, dtype=np.int32)
+      keep = [np.ascontiguousarray(out_ptr, dtype=np.int64),
+              np.ascontiguousarray(out_idx, dtype=np.int32), None, None]
+      if not symmetric:
+        keep[2] = np.ascontiguousarray(in_ptr, dtype=np.int64)
+        keep[3] = np.ascontiguousarray(in_idx, dtype=np.int32)
+      ptrs = [_ptr(a) for a in keep]
+      nnz = len(keep[1])
     h = ctypes.c_void_p()
-    _lib.check(_lib.load().cgbfs_graph_create(
-        int(device), int(n), int(len(out_idx)), _ptr(out_ptr), _ptr(out_idx), _ptr(in_ptr),
-        _ptr(in_idx), 1 if symmetric else 0, ctypes.byref(h)))
+    _lib.check(_lib.load().cgbfs_graph_create(int(device), int(n), nnz, *ptrs,
+                                              1 if symmetric else 0, ctypes.byref(h)))
+    del keep
     return cls(h, device)
 
   @classmethod
@@ -211,6 +228,7 @@ class DeviceGraph:
       par, lev = device_out
       _check_tensor(par, self.n, "int32")
       _check_tensor(lev, self.n, "int32")
+      _after_torch_stream(par)
       rc = _lib.load().cgbfs_bfs(self._h, int(source), ctypes.c_void_p(par.data_ptr()),
                                  ctypes.c_void_p(lev.data_ptr()), 1, ctypes.byref(opt),
                                  ctypes.byref(sb.s))
@@ -236,6 +254,7 @@ class DeviceGraph:
       par, lev = device_out
       _check_tensor(par, k * self.n, "int32")
       _check_tensor(lev, k * self.n, "int16")
+      _after_torch_stream(par)
       rc = _lib.load().cgbfs_msbfs(self._h, _ptr(roots), k, ctypes.c_void_p(par.data_ptr()),
                                    ctypes.c_void_p(lev.data_ptr()), 1, ctypes.byref(opt),
                                    ctypes.byref(sb.s), _ptr(m))
@@ -264,6 +283,7 @@ class DeviceGraph:
       par, lev = device_out
       _check_tensor(par, nlanes * self.n, "int32")
       _check_tensor(lev, nlanes * self.n, "int16")
+      _after_torch_stream(par)
       rc = _lib.load().cgbfs_msbfs_seeded(self._h, _ptr(verts), _ptr(lanes), len(verts), nlanes,
                                           ctypes.c_void_p(par.data_ptr()),
                                           ctypes.c_void_p(lev.data_ptr()), 1, ctypes.byref(opt),
@@ -301,6 +321,7 @@ class ServingEngine:
     graph's device; they receive this query's outputs."""
     _check_tensor(parents, self.n, "int32")
     _check_tensor(levels, self.n, "int16")
+    _after_torch_stream(parents)
     _lib.check(_lib.load().cgbfs_engine_submit(self._e, int(lane), int(root),
                                                ctypes.c_void_p(parents.data_ptr()),
                                                ctypes.c_void_p(levels.data_ptr())))
@@ -338,8 +359,17 @@ class ServingEngine:
     self.close()
 
 
+def _after_torch_stream(t):
+  """The library writes caller-provided device outputs on its own stream,
+  which does not wait for torch's streams: finish the work torch has queued
+  on this device (a fill of the same buffers, a reader of a previous result)
+  before the call enqueues its writes."""
+  import torch
+  torch.cuda.current_stream(t.device).synchronize()
+
+
 def _check_tensor(t, numel, dtype):
   import torch  # plumbing only: device memory for outputs that stay in HBM
-  want = {"int32": torch.int32, "int16": torch.int16}[dtype]
+  want = {"int64": torch.int64, "int32": torch.int32, "int16": torch.int16}[dtype]
   if t.dtype != want or not t.is_cuda or not t.is_contiguous() or t.numel() != numel:
     raise ValueError(f"expected a contiguous {dtype} CUDA tensor with {numel} elements")

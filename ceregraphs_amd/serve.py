@@ -112,10 +112,11 @@ def build_slots(specs, budget, buffers_cap=ServingEngine.LANES, **engine_opts):
                  "result_bytes": int(nbuf * per_buf)}
 
 
-def serve_stream(slots, stream, deadline_s=120.0):
+def serve_stream(slots, stream, deadline_s=120.0, on_complete=None):
   """Runs the stream through the engines; returns per-query latency (s),
   m per query, and the wall span from the first arrival to the last
-  completion."""
+  completion.  on_complete(q, (parents, levels), m, rounds), if given, sees
+  each query's result buffers before they go back to the pool."""
   nq = len(stream.t)
   lat = np.full(nq, np.nan)
   m = np.zeros(nq, np.int64)
@@ -132,7 +133,7 @@ def serve_stream(slots, stream, deadline_s=120.0):
     progressed = False
     for s in slots:
       if s.inflight:
-        ready, completed, _, mm, _ = s.engine.poll(wait=False)
+        ready, completed, _, mm, depth = s.engine.poll(wait=False)
         if not ready:
           continue
         s.inflight = False
@@ -142,6 +143,8 @@ def serve_stream(slots, stream, deadline_s=120.0):
           q, buf = s.lane_query.pop(lane)
           lat[q] = tnow - stream.t[q]
           m[q] = mm[lane]
+          if on_complete is not None:
+            on_complete(q, buf, int(mm[lane]), int(depth[lane]))
           s.free_lanes.append(lane)
           s.free_bufs.append(buf)
           done += 1

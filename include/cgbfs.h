@@ -21,6 +21,14 @@
  * Threading: calls on one graph handle are serialised by the caller and run
  * on that handle's own HIP stream; distinct handles may be used from
  * distinct threads.
+ *
+ * Device outputs: a search writes caller-provided device buffers on the
+ * handle's own stream, which does not wait for the caller's streams.  Work
+ * on other streams that writes or reads those buffers (a fill, a reader of
+ * an earlier result) must be complete before the call, e.g. by
+ * synchronising that stream.  cgbfs_bfs, cgbfs_msbfs and cgbfs_msbfs_seeded
+ * return once their outputs are complete in HBM; a serving engine's query
+ * outputs are complete once cgbfs_engine_poll reports the query completed.
  */
 #ifndef CGBFS_H
 #define CGBFS_H
@@ -109,14 +117,30 @@ CGBFS_API int cgbfs_device_count(int32_t *count);
 
 /* --- graph lifecycle (Kernel 1: construction, once per graph) ----------- */
 
-/* Copies host CSR/CSC arrays into HBM.  For symmetric != 0 the in_* arrays
- * may be NULL (the out view is shared).  Indices must be sorted ascending
- * within each in-row (pull first-hit = lowest id).  Replaces preprocess()
+/* Copies a graph into library-owned HBM.  The arrays are either all host
+ * memory or all device memory of `device` (e.g. torch tensors' data_ptr);
+ * either way the library keeps its own copy and the caller may free theirs
+ * on return.  For symmetric != 0 the in_* arrays may be NULL (the out view
+ * is shared).  The graph must pass cgbfs_graph_check: offsets run 0..nnz
+ * and never decrease, ids lie in [0, n), every row is strictly ascending
+ * (the pull takes the first frontier hit of an in-row as the lowest id),
+ * and the in view is the transpose of the out view (a symmetric graph's one
+ * view equals its transpose); otherwise CGBFS_EINVAL, before any search
+ * kernel reads the arrays.  Host arrays are checked on the host before the
+ * GPU is touched, device arrays on the device.  Replaces preprocess()
  * (bfs/implementation/preprocess_bool.py:29-229) and the matrix upload of
  * run_bfs.py:338-358. */
 CGBFS_API int cgbfs_graph_create(int32_t device, int64_t n, int64_t nnz, const int64_t *out_ptr,
                        const int32_t *out_idx, const int64_t *in_ptr, const int32_t *in_idx,
                        int32_t symmetric, cgbfs_graph **out);
+
+/* The structural check of cgbfs_graph_create on host arrays, without a GPU:
+ * CGBFS_OK, or CGBFS_EINVAL with the first defect (view, vertex or entry)
+ * in cgbfs_last_error().  The reference trusts the scipy matrices it is
+ * given (preprocess_bool.py:29-229); this has no counterpart there. */
+CGBFS_API int cgbfs_graph_check(int64_t n, int64_t nnz, const int64_t *out_ptr,
+                      const int32_t *out_idx, const int64_t *in_ptr, const int32_t *in_idx,
+                      int32_t symmetric);
 
 /* Builds an R-MAT graph on the GPU, edge-identical to
  * datasets/gen_rmat.py:15-44 for the same numpy PCG64 stream.  pcg[0..3] =

@@ -181,6 +181,7 @@ def test_msbfs_every_output_written(scale, ef, seed, directed):
     for d, cap in [(d, False) for d in DIRECTIONS + ["beamer_lanes"]] + [("beamer", True)]:
       par.fill_(0x5A5A5A5A)
       lev.fill_(0x5A5A)
+      torch.cuda.synchronize()  # the fill is on torch's stream, the search on the library's
       dg.msbfs(roots, direction=d, capture=cap, device_out=(par, lev))
       P = par.view(64, n).cpu().numpy()
       L = lev.view(64, n).cpu().numpy()
@@ -269,46 +270,101 @@ def _check_ms_stats(st, levs, direction):
     assert np.all(st.direction_history == 1)
 
 
-def test_c2_scale21_sampled_roots():
-  """Configuration 2 graph (s21 ef16 seed 1), direction-optimising searches
-  from sampled roots, bit-exact against the oracle."""
+@pytest.mark.timeout(600)
+def test_c2_scale21_all_roots():
+  """Configuration 2 (s21 ef16 seed 1, 64 roots from seed 2 among vertices
+  with degree >= 1, one direction-optimising search per launch): all 64
+  searches bit-exact against the oracle (levels, lowest-id parents, nf per
+  round, m); the 64 roots in one multi-source pass equal them lane by lane."""
   with DeviceGraph.from_rmat(21, 16, 1) as dg:
     ip, ii, op, oi = dg.download()
     g = O.Graph(dg.n, ip, ii, op, oi, True)
     deg = np.diff(op)
-    roots = np.random.default_rng(2).choice(np.nonzero(deg)[0], 3, replace=False)
-    for s in roots:
-      _check_single(dg, g, int(s), "beamer")
+    roots = np.random.default_rng(2).choice(np.nonzero(deg)[0], 64, replace=False)
+    singles = [_check_single(dg, g, int(s), "beamer") for s in roots]
     _check_single(dg, g, int(roots[0]), "tau")
-    r = dg.msbfs(roots)
-    for l, s in enumerate(roots):
-      single = dg.bfs(int(s))
-      np.testing.assert_array_equal(r.levels[l], single.levels.astype(np.int16))
-      np.testing.assert_array_equal(r.parents[l], single.parents)
+    r = dg.msbfs(roots, capture=True)
+    for l, single in enumerate(singles):
+      np.testing.assert_array_equal(r.levels[l], single.levels.astype(np.int16), err_msg=f"lane {l}")
+      np.testing.assert_array_equal(r.parents[l], single.parents, err_msg=f"lane {l}")
+      assert r.m_lanes[l] == single.stats.m_edges
 
 
+@pytest.mark.timeout(600)
 def test_c3_scale24_msbfs_lanes_equal_single_root():
-  """Configuration 3 graph (s24 ef16 seed 3), 64 roots from seed 4 in one
-  multi-source pass, each lane bit-exact to the single-root search; two
-  lanes checked against the oracle in full."""
+  """Configuration 3 (s24 ef16 seed 3, 64 roots from seed 4): the pass the
+  benchmark times (graph replay, every level decided on the device) and the
+  polled, instrumented pass both give, lane by lane, exactly the 64
+  single-root searches; two lanes are checked against the oracle in full."""
   import torch
   with DeviceGraph.from_rmat(24, 16, 3) as dg:
     n = dg.n
     deg = dg.out_degree()
     roots = np.random.default_rng(4).choice(np.nonzero(deg > 0)[0], 64, replace=False)
-    par = torch.empty(64 * n, dtype=torch.int32, device="cuda")
-    lev = torch.empty(64 * n, dtype=torch.int16, device="cuda")
-    r = dg.msbfs(roots, device_out=(par, lev))
+    outs = {}
+    for cap in (True, False):
+      par = torch.empty(64 * n, dtype=torch.int32, device="cuda")
+      lev = torch.empty(64 * n, dtype=torch.int16, device="cuda")
+      r = dg.msbfs(roots, device_out=(par, lev), capture=cap)
+      outs[cap] = (par, lev, r.m_lanes)
     par2 = torch.empty(n, dtype=torch.int32, device="cuda")
     lev2 = torch.empty(n, dtype=torch.int32, device="cuda")
     for l in range(64):
       single = dg.bfs(int(roots[l]), device_out=(par2, lev2))
-      assert torch.equal(lev[l * n:(l + 1) * n], lev2.to(torch.int16)), f"lane {l}"
-      assert torch.equal(par[l * n:(l + 1) * n], par2), f"lane {l}"
-      assert r.m_lanes[l] == single.stats.m_edges
+      for cap, (par, lev, m_lanes) in outs.items():
+        assert torch.equal(lev[l * n:(l + 1) * n], lev2.to(torch.int16)), f"lane {l} capture {cap}"
+        assert torch.equal(par[l * n:(l + 1) * n], par2), f"lane {l} capture {cap}"
+        assert m_lanes[l] == single.stats.m_edges
     ip, ii, op, oi = dg.download()
     g = O.Graph(n, ip, ii, op, oi, True)
+    par, lev, _ = outs[True]
     for l in (0, 63):
       lev0, par0, _, _ = O.bfs(g, int(roots[l]))
       np.testing.assert_array_equal(lev[l * n:(l + 1) * n].cpu().numpy(), lev0.astype(np.int16))
       np.testing.assert_array_equal(par[l * n:(l + 1) * n].cpu().numpy(), par0)
+
+
+def test_graph_from_device_arrays():
+  """A graph handed over as device arrays (torch tensors) is copied device to
+  device, checked on the GPU, and searches exactly like the same graph from
+  host arrays; the caller's tensors may be freed right after."""
+  import torch
+  for directed in (False, True):
+    abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+    n, ip, ii, op, oi = gen_rmat.rmat_csr(12, 8, 7, *abcd[:3], directed=directed)
+    g = O.Graph(n, ip, ii, op, oi, not directed)
+    t = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (op, oi, ip, ii)]
+    dg = DeviceGraph.from_csr(n, *t, symmetric=not directed)
+    del t
+    torch.cuda.empty_cache()
+    with dg:
+      gip, gii, gop, goi = dg.download()
+      np.testing.assert_array_equal(gii, ii)
+      np.testing.assert_array_equal(goi, oi)
+      for s in (0, int(np.argmax(np.diff(op)))):
+        _check_single(dg, g, s, "beamer")
+
+
+@pytest.mark.parametrize("defect", ["id", "order", "ptr", "transpose"])
+def test_graph_from_device_arrays_rejects_malformed(defect):
+  """The device check catches each defect class before any search kernel
+  reads the graph (CGBFS_EINVAL, no handle)."""
+  import torch
+  from ceregraphs_amd import _lib
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(11, 8, 9)
+  ii = ii.copy()
+  ip = ip.copy()
+  v = int(np.argmax(np.diff(ip)))
+  if defect == "id":
+    ii[ip[v] + 1] = n + 5
+  elif defect == "order":
+    ii[ip[v]], ii[ip[v] + 1] = ii[ip[v] + 1], ii[ip[v]]
+  elif defect == "ptr":
+    ip[v + 1] = ip[v] - 1
+  else:
+    ii[ip[v + 1] - 1] = n - 1 if ii[ip[v + 1] - 1] != n - 1 else n - 2
+    ii[ip[v]:ip[v + 1]].sort()
+  t = [torch.from_numpy(a).cuda() for a in (ip, ii)]
+  with pytest.raises(_lib.CgbfsError) as err:
+    DeviceGraph.from_csr(n, *t, symmetric=True)
+  assert err.value.code == _lib.CGBFS_EINVAL

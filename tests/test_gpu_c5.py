@@ -1,0 +1,85 @@
+"""Serving at its real size on the GPU: six undirected Graph500 R-MAT graphs
+(s16, s18, s20, s22, s24, s26, ef16, seeds 21-26) resident together, one
+serving engine each (continuous batching), result pools sized within the
+200 GB HBM budget, and a Poisson stream of queries at 1000 queries/s (seed
+20), each picking a graph and then a non-isolated root uniformly.
+
+Every query's parents and levels are kept as it completes.  At s16-s22 each
+query is checked against the oracle (levels, lowest-id parents, m, rounds);
+at s24 and s26 each query equals the same query run alone (cgbfs_bfs), and
+two queries per graph are also checked against the oracle.  The graphs are
+built and the stream served once (module fixture); each graph is then its
+own test."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ceregraphs_amd import serve
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
+
+QUERIES = 240
+
+
+@pytest.fixture(scope="module")
+def c5():
+  import torch
+  slots, mem = serve.build_slots(serve.SERVED_SPECS, serve.HBM_BUDGET)
+  nonisolated = [np.nonzero(s.dg.out_degree() > 0)[0] for s in slots]
+  stream = serve.poisson_stream(1000.0, QUERIES, nonisolated, seed=serve.STREAM_SEED)
+  kept = {}
+
+  def keep(q, buf, m, rounds):
+    kept[q] = (buf[0].clone(), buf[1].clone(), m, rounds)
+
+  lat, m, span = serve.serve_stream(slots, stream, deadline_s=300.0, on_complete=keep)
+  torch.cuda.synchronize()
+  yield {"slots": slots, "stream": stream, "kept": kept, "lat": lat, "m": m, "mem": mem}
+  for s in slots:
+    s.close()
+    s.dg.close()
+
+
+def test_c5_stream_completed(c5):
+  st = c5["stream"]
+  assert len(c5["kept"]) == QUERIES and np.isfinite(c5["lat"]).all()
+  assert c5["mem"]["result_buffers_per_graph"] >= 1
+  assert c5["mem"]["graphs_and_engines_bytes"] + c5["mem"]["result_bytes"] <= serve.HBM_BUDGET
+  for q in range(QUERIES):
+    assert c5["kept"][q][2] == c5["m"][q]
+  # every graph was queried
+  assert set(st.graph.tolist()) == set(range(len(serve.SERVED_SPECS)))
+
+
+def _oracle_graph(dg):
+  ip, ii, op, oi = dg.download()
+  return O.Graph(dg.n, ip, ii, op, oi, True)
+
+
+@pytest.mark.parametrize("gi", range(len(serve.SERVED_SPECS)))
+def test_c5_queries(c5, gi):
+  """s16-s22: every query == the oracle.  s24, s26: every query == the
+  same query alone, two of them == the oracle."""
+  st, kept = c5["stream"], c5["kept"]
+  dg = c5["slots"][gi].dg
+  scale = serve.SERVED_SPECS[gi][0]
+  qs = np.nonzero(st.graph == gi)[0]
+  assert len(qs) > 0
+  g = _oracle_graph(dg)
+  oracle_qs = qs if scale <= 22 else qs[:2]
+  for q in oracle_qs:
+    par, lev, m, rounds = kept[int(q)]
+    lev0, par0, _, rounds0 = O.bfs(g, int(st.root[q]))
+    np.testing.assert_array_equal(lev.cpu().numpy(), lev0.astype(np.int16), err_msg=f"query {q}")
+    np.testing.assert_array_equal(par.cpu().numpy(), par0, err_msg=f"query {q}")
+    assert (m, rounds) == (O.m_edges(g, lev0), rounds0), f"query {q}"
+  if scale >= 24:
+    import torch
+    p1 = torch.empty(dg.n, dtype=torch.int32, device="cuda")
+    l1 = torch.empty(dg.n, dtype=torch.int32, device="cuda")
+    for q in qs:
+      par, lev, m, _ = kept[int(q)]
+      alone = dg.bfs(int(st.root[q]), device_out=(p1, l1))
+      assert torch.equal(lev, l1.to(torch.int16)), f"query {q}"
+      assert torch.equal(par, p1), f"query {q}"
+      assert m == alone.stats.m_edges, f"query {q}"
