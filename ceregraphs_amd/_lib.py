@@ -30,7 +30,8 @@ EXPORTED = (
     "cgbfs_last_error", "cgbfs_version", "cgbfs_device_count", "cgbfs_graph_create",
     "cgbfs_graph_check",
     "cgbfs_graph_create_rmat", "cgbfs_graph_info", "cgbfs_graph_download",
-    "cgbfs_graph_out_degree", "cgbfs_graph_destroy", "cgbfs_bfs", "cgbfs_msbfs",
+    "cgbfs_graph_out_degree", "cgbfs_graph_destroy", "cgbfs_bfs", "cgbfs_bfs_noalloc",
+    "cgbfs_msbfs",
     "cgbfs_msbfs_seeded", "cgbfs_sync", "cgbfs_engine_create", "cgbfs_engine_submit",
     "cgbfs_engine_step", "cgbfs_engine_poll", "cgbfs_engine_destroy",
 )
@@ -103,6 +104,8 @@ _SIGNATURES = {
     "cgbfs_graph_destroy": (ctypes.c_int, [_GP]),
     "cgbfs_bfs": (ctypes.c_int, [_GP, ctypes.c_int64, _P, _P, ctypes.c_int32,
                                  ctypes.POINTER(Options), ctypes.POINTER(Stats)]),
+    "cgbfs_bfs_noalloc": (ctypes.c_int, [_GP, ctypes.c_int64, _P, _P, ctypes.POINTER(Options),
+                                         ctypes.POINTER(Stats)]),
     "cgbfs_msbfs": (ctypes.c_int, [_GP, _P, ctypes.c_int32, _P, _P, ctypes.c_int32,
                                    ctypes.POINTER(Options), ctypes.POINTER(Stats), _P]),
     "cgbfs_msbfs_seeded": (ctypes.c_int, [_GP, _P, _P, ctypes.c_int64, ctypes.c_int32, _P, _P,

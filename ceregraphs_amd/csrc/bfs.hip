@@ -21,6 +21,7 @@
 // level d-1 bitmap, no longer read), so every bitmap is clear one round
 // before it is written.
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cg_common.h"
@@ -91,24 +92,25 @@ struct Workspace {
   unsigned long long *cursor = nullptr;   // [kParts * kCursorStride] pull chunk cursors
 };
 
-static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+static constexpr size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static int ensure_ws1(DeviceGraph *g, Workspace *w) {
-  const int64_t n = g->n;
+// Size of the single-root workspace of an n-vertex graph.  internal_out
+// adds the int32 levels / parents a search writes when the caller's outputs
+// are host memory (copied out at the end).
+static constexpr size_t ws1_size(int64_t n, bool internal_out) {
+  return align_up(sizeof(uint32_t) * (size_t)(((n + 63) / 64) * 2)) * 4 +  // visited, fb[3]
+         align_up(sizeof(int32_t) * (size_t)n) * 3 +                       // q[2], bigq
+         align_up(sizeof(int64_t) * (size_t)(n + 1)) +                      // bigpre
+         (internal_out ? align_up(sizeof(int32_t) * (size_t)n) * 2 : 0) +   // levels, parents
+         align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64) +
+         align_up(sizeof(unsigned long long) * kParts * kCursorStride);
+}
+
+// Lays the workspace of ws1_size(n, internal_out) bytes out from `base`.
+static void ws1_layout(int64_t n, bool internal_out, char *p, Workspace *w) {
   const int64_t nwords = ((n + 63) / 64) * 2;
-  size_t bm = align_up(sizeof(uint32_t) * nwords);
-  size_t qb = align_up(sizeof(int32_t) * n);
-  size_t total = bm * 4 + qb * 3 + align_up(sizeof(int64_t) * (n + 1)) + align_up(sizeof(int32_t) * n) + align_up(sizeof(uint32_t) * n) +
-                 align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64) +
-                 align_up(sizeof(unsigned long long) * kParts * kCursorStride);
-  if (g->ws1_bytes < total) {
-    if (g->ws1) hipFree(g->ws1);
-    g->ws1 = nullptr;
-    g->ws1_bytes = 0;
-    CG_HIP(hipMalloc(&g->ws1, total));
-    g->ws1_bytes = total;
-  }
-  char *p = (char *)g->ws1;
+  const size_t bm = align_up(sizeof(uint32_t) * nwords);
+  const size_t qb = align_up(sizeof(int32_t) * n);
   w->n = n;
   w->nwords = nwords;
   w->visited = (uint32_t *)p; p += bm;
@@ -116,12 +118,61 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
   for (int i = 0; i < 2; i++) { w->q[i] = (int32_t *)p; p += qb; }
   w->bigq = (int32_t *)p; p += qb;
   w->bigpre = (int64_t *)p; p += align_up(sizeof(int64_t) * (n + 1));
-  w->levels = (int32_t *)p; p += align_up(sizeof(int32_t) * n);
-  w->parents = (uint32_t *)p; p += align_up(sizeof(uint32_t) * n);
+  if (internal_out) {
+    w->levels = (int32_t *)p; p += align_up(sizeof(int32_t) * n);
+    w->parents = (uint32_t *)p; p += align_up(sizeof(uint32_t) * n);
+  } else {
+    w->levels = nullptr;
+    w->parents = nullptr;
+  }
   w->ctrl = (Ctrl *)p; p += align_up(sizeof(Ctrl));
   w->hist = (Hist *)p; p += align_up(sizeof(Hist));
   w->scalars = (unsigned long long *)p; p += align_up(64);
   w->cursor = (unsigned long long *)p;
+}
+
+// The graph's own workspace (cgbfs_bfs), allocated on first use and grown
+// when a search needs more.
+static int ensure_ws1(DeviceGraph *g, Workspace *w) {
+  const size_t total = ws1_size(g->n, true);
+  if (g->ws1_bytes < total) {
+    if (g->ws1) hipFree(g->ws1);
+    g->ws1 = nullptr;
+    g->ws1_bytes = 0;
+    CG_HIP(hipMalloc(&g->ws1, total));
+    g->ws1_bytes = total;
+  }
+  ws1_layout(g->n, true, (char *)g->ws1, w);
+  return CGBFS_OK;
+}
+
+// The workspace of cgbfs_bfs_noalloc: one file-scope device array per GPU,
+// sized for kNoAllocMaxN vertices and held until the process ends.  Host
+// code never uses the array's name as an address (on the host it is only a
+// stand-in for the symbol): its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+constexpr int64_t kNoAllocMaxN = (int64_t)1 << 21;
+constexpr size_t kNoAllocBytes = ws1_size(kNoAllocMaxN, false);
+static_assert(kNoAllocBytes == 43140864, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices];
+
+// The current device's address of `scratch` (the caller made `device`
+// current).
+static int scratch_address(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return set_error(CGBFS_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    if (!ptr) return set_error(CGBFS_EHIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_dev[device] = ptr;
+  }
+  *out = g_scratch_dev[device];
   return CGBFS_OK;
 }
 
@@ -204,6 +255,12 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, BfsParams P, long long t
   st_ctrl(&c->done, 0);
   st_ctrl(&c->status, 0);
   reset_cursors(A.cursor);
+}
+
+// The search's start time, before anything else of it runs on the stream;
+// every decide then records its own time in t_end.
+__global__ void bfs_start(Ctrl *c) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const unsigned long long t0 = wall_clock64();
   st_ctrl(&c->t0, t0);
   st_ctrl(&c->t_end, t0);
@@ -662,29 +719,19 @@ void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o) {
 
 using namespace cg;
 
-extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32_t *levels,
-                         int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats) {
-  if (!h) return set_error(CGBFS_EINVAL, "graph is null");
-  DeviceGraph *g = &h->g;
-  if (source < 0 || source >= g->n) return set_error(CGBFS_EINVAL, "source out of range");
-  if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
-  cgbfs_options o;
-  fill_options(opt, g->n, &o);
-  if (o.direction < 0 || o.direction > CGBFS_DIR_BEAMER_LANES)
-    return set_error(CGBFS_EINVAL, "unknown direction mode");
-  if (o.direction == CGBFS_DIR_BEAMER_LANES) o.direction = CGBFS_DIR_BEAMER;  // one lane
-  CG_HIP(hipSetDevice(g->device));
-  Workspace ws;
-  int rc = ensure_ws1(g, &ws);
-  if (rc) return rc;
-
+// The search itself, shared by cgbfs_bfs and cgbfs_bfs_noalloc: outputs go
+// to device arrays par / lev (the caller's or the workspace's).  events: time
+// each round's traversal kernels with HIP events; without them a round's time
+// is the span between its decide and the one before (device clock).
+static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t *lev,
+                   uint32_t *par, const cgbfs_options &o, cgbfs_stats *stats, bool events) {
   LevelArgs A;
   A.out_ptr = g->out_ptr;
   A.out_idx = g->out_idx;
   A.in_ptr = g->in_ptr;
   A.in_idx = g->in_idx;
-  A.levels = on_device ? levels : ws.levels;
-  A.parents = on_device ? (uint32_t *)parents : ws.parents;
+  A.levels = lev;
+  A.parents = par;
   A.visited = ws.visited;
   for (int i = 0; i < 3; i++) A.fb[i] = ws.fb[i];
   A.q[0] = ws.q[0];
@@ -700,10 +747,6 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   hipStream_t st = g->stream;
   const int blocks = grid_blocks(g->device, o.grid_mult);
   EventPool pool;
-  hipEvent_t e0 = pool.make(), e1 = pool.make();
-  if (!e0 || !e1) return set_error(CGBFS_EHIP, "hipEventCreate failed");
-  CG_HIP(hipEventRecord(e0, st));
-  bfs_init_fill<<<blocks, kBlock, 0, st>>>(A);
   BfsParams P;
   P.mode = o.direction;
   P.alpha = o.alpha;
@@ -711,23 +754,27 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
   P.tau = (long long)o.tau;
   P.max_levels = (long long)o.max_levels;
   P.n_in = (long long)g->n_in;
+  bfs_start<<<1, 64, 0, st>>>(A.ctrl);
+  bfs_init_fill<<<blocks, kBlock, 0, st>>>(A);
   bfs_init_source<<<1, 64, 0, st>>>(A, source, P, (long long)g->total_out_edges);
   CG_HIP(hipGetLastError());
 
   // Rounds are enqueued in chunks without host synchronisation; kernels of
   // rounds past the end see ctrl->done and return at once.  The host polls
-  // the done flag once per chunk; e1 is re-recorded after every chunk so it
-  // ends up right after the chunk in which the search finished.
+  // the done flag once per chunk.  The search time is the device clock from
+  // bfs_start to the decide of the last round (ctrl->t0 .. ctrl->t_end), so
+  // neither the gated launches after the end nor the host poll count.
   int64_t d = 0;
   int chunk = o.rounds_hint > 0 ? o.rounds_hint : 12;
   Ctrl hc;
-  std::vector<hipEvent_t> rev;  // two per round: around its traversal kernels
+  hipEvent_t rev[2 * 64];  // per-round events of the first 64 rounds
+  int nrev = 0;
   for (;;) {
     for (int k = 0; k < chunk; k++, d++) {
       hipEvent_t ra = nullptr, rb = nullptr;
-      if (d < kHistCap && (ra = pool.make()) && (rb = pool.make())) {
-        rev.push_back(ra);
-        rev.push_back(rb);
+      if (events && nrev + 2 <= 2 * 64 && (ra = pool.make()) && (rb = pool.make())) {
+        rev[nrev++] = ra;
+        rev[nrev++] = rb;
         hipEventRecord(ra, st);
       }
       bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
@@ -738,51 +785,48 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
       bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n, P);
     }
     CG_HIP(hipGetLastError());
-    CG_HIP(hipEventRecord(e1, st));
     CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     CG_HIP(hipStreamSynchronize(st));
     if (hc.done) break;
     if (d > o.max_levels + chunk) break;
     chunk = 4;
   }
-  float ms = 0;
-  CG_HIP(hipEventElapsedTime(&ms, e0, e1));
   if (hc.status) return set_error(hc.status, "BFS exceeded max_levels");
-
-  double out_ms = 0;
-  if (!on_device) {
-    hipEvent_t o0 = pool.make(), o1 = pool.make();
-    if (!o0 || !o1) return set_error(CGBFS_EHIP, "hipEventCreate failed");
-    CG_HIP(hipEventRecord(o0, st));
-    CG_HIP(hipMemcpyAsync(parents, A.parents, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
-    CG_HIP(hipMemcpyAsync(levels, A.levels, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
-    CG_HIP(hipEventRecord(o1, st));
-    CG_HIP(hipEventSynchronize(o1));
-    float oms = 0;
-    hipEventElapsedTime(&oms, o0, o1);
-    out_ms = oms;
-  }
+  const double hz = wall_clock_hz(g->device);
 
   if (stats) {
     stats->rounds_completed = hc.rounds;
-    stats->total_ms = ms;
-    stats->output_ms = out_ms;
+    stats->total_ms = (double)(hc.t_end - hc.t0) / hz * 1e3;
+    stats->output_ms = 0;
     int64_t nh = hc.rounds < kHistCap ? hc.rounds : kHistCap;
     if (stats->max_hist < nh) nh = stats->max_hist;
     if (nh > 0) {
-      std::vector<char> hb(sizeof(Hist));
-      CG_HIP(hipMemcpy(hb.data(), A.hist, sizeof(Hist), hipMemcpyDeviceToHost));
-      const Hist *H = (const Hist *)hb.data();
-      for (int64_t r = 0; r < nh; r++) {
-        if (stats->nf_history) stats->nf_history[r] = H->nf[r];
-        if (stats->direction_history) stats->direction_history[r] = H->dir[r];
-        if (stats->edges_scanned) stats->edges_scanned[r] = H->scanned[r];
-        if (stats->rows_scanned) stats->rows_scanned[r] = H->rows[r];
-        if (stats->pairs_found) stats->pairs_found[r] = H->nf[r];
-        if (stats->level_ms && 2 * r + 1 < (int64_t)rev.size()) {
-          float rm = 0;
-          hipEventElapsedTime(&rm, rev[2 * r], rev[2 * r + 1]);
-          stats->level_ms[r] = rm;
+      // history slices straight into the caller's arrays (long long and
+      // int64_t, int and int32_t have the same layout here)
+      const size_t b8 = sizeof(long long) * (size_t)nh;
+      if (stats->nf_history) CG_HIP(hipMemcpy(stats->nf_history, A.hist->nf, b8, hipMemcpyDeviceToHost));
+      if (stats->pairs_found) CG_HIP(hipMemcpy(stats->pairs_found, A.hist->nf, b8, hipMemcpyDeviceToHost));
+      if (stats->edges_scanned)
+        CG_HIP(hipMemcpy(stats->edges_scanned, A.hist->scanned, b8, hipMemcpyDeviceToHost));
+      if (stats->rows_scanned) CG_HIP(hipMemcpy(stats->rows_scanned, A.hist->rows, b8, hipMemcpyDeviceToHost));
+      if (stats->direction_history)
+        CG_HIP(hipMemcpy(stats->direction_history, A.hist->dir, sizeof(int) * nh, hipMemcpyDeviceToHost));
+      if (stats->level_ms) {
+        // the decide times land in the caller's array as raw clock values
+        // first; each is turned into its round's span in place
+        CG_HIP(hipMemcpy(stats->level_ms, A.hist->t, sizeof(unsigned long long) * nh,
+                         hipMemcpyDeviceToHost));
+        unsigned long long prev = hc.t0;
+        for (int64_t r = 0; r < nh; r++) {
+          unsigned long long t = 0;
+          std::memcpy(&t, &stats->level_ms[r], sizeof(t));
+          double ms = (double)(t - prev) / hz * 1e3;
+          prev = t;
+          if (2 * r + 1 < nrev) {
+            float rm = 0;
+            if (hipEventElapsedTime(&rm, rev[2 * r], rev[2 * r + 1]) == hipSuccess) ms = rm;
+          }
+          stats->level_ms[r] = ms;
         }
       }
     }
@@ -796,4 +840,63 @@ extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32
     stats->visited_count = (int64_t)sc[1];
   }
   return CGBFS_OK;
+}
+
+static int bfs_args(cgbfs_graph *h, int64_t source, const void *parents, const void *levels,
+                    const cgbfs_options *opt, cgbfs_options *o) {
+  if (!h) return set_error(CGBFS_EINVAL, "graph is null");
+  if (source < 0 || source >= h->g.n) return set_error(CGBFS_EINVAL, "source out of range");
+  if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
+  fill_options(opt, h->g.n, o);
+  if (o->direction < 0 || o->direction > CGBFS_DIR_BEAMER_LANES)
+    return set_error(CGBFS_EINVAL, "unknown direction mode");
+  if (o->direction == CGBFS_DIR_BEAMER_LANES) o->direction = CGBFS_DIR_BEAMER;  // one lane
+  return CGBFS_OK;
+}
+
+extern "C" int cgbfs_bfs(cgbfs_graph *h, int64_t source, int32_t *parents, int32_t *levels,
+                         int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats) {
+  cgbfs_options o;
+  int rc = bfs_args(h, source, parents, levels, opt, &o);
+  if (rc) return rc;
+  DeviceGraph *g = &h->g;
+  CG_HIP(hipSetDevice(g->device));
+  Workspace ws;
+  rc = ensure_ws1(g, &ws);
+  if (rc) return rc;
+  int32_t *lev = on_device ? levels : ws.levels;
+  uint32_t *par = on_device ? (uint32_t *)parents : ws.parents;
+  rc = bfs_run(g, ws, source, lev, par, o, stats, true);
+  if (rc || on_device) return rc;
+  EventPool pool;
+  hipEvent_t o0 = pool.make(), o1 = pool.make();
+  if (!o0 || !o1) return set_error(CGBFS_EHIP, "hipEventCreate failed");
+  hipStream_t st = g->stream;
+  CG_HIP(hipEventRecord(o0, st));
+  CG_HIP(hipMemcpyAsync(parents, par, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
+  CG_HIP(hipMemcpyAsync(levels, lev, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
+  CG_HIP(hipEventRecord(o1, st));
+  CG_HIP(hipEventSynchronize(o1));
+  float oms = 0;
+  hipEventElapsedTime(&oms, o0, o1);
+  if (stats) stats->output_ms = oms;
+  return CGBFS_OK;
+}
+
+extern "C" int cgbfs_bfs_noalloc(cgbfs_graph *h, int64_t source, int32_t *parents,
+                                 int32_t *levels, const cgbfs_options *opt, cgbfs_stats *stats) {
+  cgbfs_options o;
+  int rc = bfs_args(h, source, parents, levels, opt, &o);
+  if (rc) return rc;
+  DeviceGraph *g = &h->g;
+  if (g->n > kNoAllocMaxN)
+    return set_error(CGBFS_EINVAL, "cgbfs_bfs_noalloc handles graphs of at most 2^21 vertices; "
+                                   "use cgbfs_bfs for larger ones");
+  CG_HIP(hipSetDevice(g->device));
+  void *base = nullptr;
+  rc = scratch_address(g->device, &base);
+  if (rc) return rc;
+  Workspace ws;
+  ws1_layout(g->n, false, (char *)base, &ws);
+  return bfs_run(g, ws, source, levels, (uint32_t *)parents, o, stats, false);
 }

@@ -176,6 +176,23 @@ CGBFS_API int cgbfs_graph_destroy(cgbfs_graph *g);
 CGBFS_API int cgbfs_bfs(cgbfs_graph *g, int64_t source, int32_t *parents, int32_t *levels,
               int32_t on_device, const cgbfs_options *opt, cgbfs_stats *stats);
 
+/* cgbfs_bfs with device outputs that allocates nothing: no hipMalloc,
+ * hipFree, event or other allocating call, for callers that cannot allocate
+ * (code run under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  No counterpart in the reference.  Its workspace (the
+ * frontier bitmaps, queues, control block and round history, 43,140,864
+ * bytes) is one file-scope device array per GPU, held from library load
+ * until the process ends and sized for graphs of at most 2^21 vertices:
+ * larger graphs get CGBFS_EINVAL (use cgbfs_bfs).  parents/levels: int32[n]
+ * device arrays on the graph's device.  Results and statistics equal
+ * cgbfs_bfs's for the same source and options; level_ms comes from the
+ * device clock (decide to decide), not from events.  Every call on one GPU
+ * uses that GPU's one array (each search clears what it needs at its
+ * start), so calls that overlap on one GPU -- from different graph handles
+ * or threads -- race and must be serialised by the caller. */
+CGBFS_API int cgbfs_bfs_noalloc(cgbfs_graph *g, int64_t source, int32_t *parents, int32_t *levels,
+                      const cgbfs_options *opt, cgbfs_stats *stats);
+
 /* Multi-source BFS: nroots <= 64 roots packed as the bits of one uint64 word
  * per vertex.  parents: int32[nroots][n]; levels: int16[nroots][n]; lane l
  * is exactly cgbfs_bfs(roots[l]) (levels cast to int16; a search deeper
