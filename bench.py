@@ -3,19 +3,33 @@
 with 64 roots -- batched multi-source BFS against the same roots run one by
 one -- on one MI355X.
 
-A step is one 64-lane multi-source pass over the 64 roots, inputs (graph)
-already resident in HBM, outputs parents int32[64][V] and levels int16[64][V]
-written to HBM.  GTEPS accounting follows the reference: m per search is the
-undirected_dedup count of bfs/implementation/bfs_timing.py:390-392, roots are
-drawn as in bfs/scripts/run_graph500.py:106-127 (non-isolated vertices,
+    python bench.py                      # the metric (s24 ef16, 64 roots)
+    python bench.py --workload c2        # s21 ef16 seed 1, 64 roots one per launch
+    python bench.py --workload c1        # s10 ef16 seed 0 balanced 4x4, source 0
+
+A step is one 64-lane multi-source pass over the 64 roots (c3), or one
+single-root search (c2: each of the 64 roots in turn; c1: source 0), with
+the graph already resident in HBM and parents / levels written to HBM.
+GTEPS accounting follows the reference: m per search is the undirected_dedup
+count of bfs/implementation/bfs_timing.py:390-392, roots are drawn as in
+bfs/scripts/run_graph500.py:106-127 (non-isolated vertices,
 default_rng(seed).choice without replacement) and the mean is the harmonic
 mean of run_graph500.py:130-134.
+
+Beside the GPU line the reference's CPU path is timed on this host's cores:
+the oracle's literal boolean-SpMV restatement of the round loop
+(bool_pe.csl:865-1004) and scipy's breadth_first_order, the reference's own
+correctness oracle (run_bfs.py:513-552), both single-threaded.  The
+Cerebras simulator run (cs_python run_bfs.py) is recorded as unavailable
+when the SDK is absent.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
 import time
 
@@ -25,6 +39,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
+METRIC_C3 = "harmonic-mean GTEPS, RMAT s24 ef16, 64 roots: batched MS-BFS vs per-root"
+WORKLOADS = {
+    "c3": {"scale": 24, "edgefactor": 16, "seed": 3, "root_seed": 4, "nroots": 64},
+    "c2": {"scale": 21, "edgefactor": 16, "seed": 1, "root_seed": 2, "nroots": 64},
+    "c1": {"scale": 10, "edgefactor": 16, "seed": 0, "root_seed": 2, "nroots": 64,
+           "balance_grid": 4, "source": 0},
+}
 
 
 def log(msg):
@@ -47,23 +68,179 @@ def pull_round_bytes(n, rows, edges, pairs, nf, noin_pairs=0):
   return 8 * n + 8 * rows + 12 * edges + 6 * pairs + 16 * nf + 6 * noin_pairs
 
 
+def level_bytes(n, direction, edges, rows, nf, frontier):
+  """Algorithmic HBM bytes of one single-root level.  Pull: visited and
+  frontier bitmaps (V/8 each), the offsets of every row opened (8 U), 4 B
+  per adjacency entry read (4 E) and parent + level of each new vertex
+  (8 N).  Push: the frontier queue (4 F), its offsets (8 F), 4 E, 8 N and
+  the next queue (4 N)."""
+  if direction == 1:
+    return n / 8 + n / 8 + 8 * rows + 4 * edges + 8 * nf
+  return 12 * frontier + 4 * edges + 12 * nf
+
+
+def per_root_roofline(n, results):
+  """The dominant level of each single-root search (its longest one, a pull
+  level on these graphs) over all searches: algorithmic bytes from the
+  level's device counters / the level's event time (its traversal kernels,
+  bfs_push_round .. bfs_pull_round)."""
+  byts, secs, dirs = 0.0, 0.0, []
+  for r in results:
+    st = r.stats
+    d = int(np.argmax(st.level_ms))
+    front = 1 if d == 0 else int(st.nf_history[d - 1])
+    byts += level_bytes(n, int(st.direction_history[d]), int(st.edges_scanned[d]),
+                        int(st.rows_scanned[d]), int(st.nf_history[d]), front)
+    secs += float(st.level_ms[d]) / 1e3
+    dirs.append(int(st.direction_history[d]))
+  achieved = byts / secs / 1e9 if secs > 0 else 0.0
+  return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+          "frac": round(achieved / PEAK_HBM_GBS, 4), "kernel": "bfs_pull_round",
+          "level": "longest level of each search",
+          "pull_levels": int(sum(dirs)), "searches": len(results),
+          "avg_level_ms": round(secs * 1e3 / max(len(results), 1), 4),
+          "algorithmic_bytes_per_level": int(byts / max(len(results), 1))}
+
+
+def traffic_from_profile(name, key="hbm_bytes_per_launch"):
+  """Measured HBM bytes per launch from a committed rocprof summary
+  (profiles/<name>), with the commit its kernels were profiled at."""
+  path = os.path.join(HERE, "profiles", name)
+  if not os.path.exists(path):
+    return None, None
+  with open(path, encoding="utf-8") as f:
+    prof = json.load(f)
+  return prof.get(key), {"profile": f"profiles/{name}", "commit": prof.get("commit")}
+
+
+def host_info():
+  model = "unknown"
+  try:
+    with open("/proc/cpuinfo", encoding="utf-8") as f:
+      for line in f:
+        if line.startswith("model name"):
+          model = line.split(":", 1)[1].strip()
+          break
+  except OSError:
+    pass
+  try:
+    avail = len(os.sched_getaffinity(0))
+  except AttributeError:
+    avail = os.cpu_count()
+  return {"cpu_model": model, "nproc": avail}
+
+
+def csl_simulator_status():
+  """The reference's own CPU path is the Cerebras fabric simulator
+  (cs_python bfs/scripts/run_bfs.py); it needs the Cerebras SDK."""
+  if shutil.which("cs_python") and shutil.which("cslc"):
+    return {"status": "available, not run", "reason": "cs_python found"}
+  return {"status": "unavailable",
+          "reason": "the Cerebras SDK (cslc, cs_python) is not installed on this host"}
+
+
+def cpu_baseline(dg, roots, spmv_roots, bfo_roots, lanes_check=None):
+  """Times the oracle's boolean-SpMV BFS (spmv_roots of the roots) and scipy
+  breadth_first_order (bfo_roots) on one host core over the same graph;
+  matrix construction stays outside the timed region, as the graph upload
+  does on the GPU.  lanes_check(l) -> int16/int32 levels of root l from the
+  GPU, compared with the CPU levels."""
+  from oracle import oracle as O
+  from scipy.sparse.csgraph import breadth_first_order
+  n = dg.n
+  ip, ii, op, oi = dg.download()
+  g = O.Graph(n, ip, ii, op, oi, dg.symmetric)
+  out = {"unit": "GTEPS", "cores": 1, "kind": "port"}
+  if spmv_roots:
+    M = g.matrix().astype(np.int32)
+    t, m, bad = 0.0, 0, 0
+    for l in range(spmv_roots):
+      s = int(roots[l])
+      tc = time.perf_counter()
+      lev, _, _, _ = O.spmv_bfs(g, s, matrix=M)
+      t += time.perf_counter() - tc
+      m += O.m_edges(g, lev)
+      if lanes_check is not None and not np.array_equal(lanes_check(l), lev):
+        bad += 1
+      log(f"cpu spmv root {l}: {t:.1f}s so far")
+    out.update({"value": round(m / t / 1e9, 5),
+                "sample": f"{spmv_roots} of {len(roots)} roots, oracle.spmv_bfs (the round loop "
+                          f"as scipy boolean SpMV) over the same graph, {t:.1f}s",
+                "levels_differ_from_gpu": bad})
+    del M
+  if bfo_roots:
+    fwd = g.matrix().transpose().tocsr()
+    t, m = 0.0, 0
+    for l in range(bfo_roots):
+      s = int(roots[l])
+      tc = time.perf_counter()
+      order = breadth_first_order(fwd, s, directed=True, return_predecessors=False)
+      t += time.perf_counter() - tc
+      visited = np.zeros(n, bool)
+      visited[order] = True
+      lev = np.where(visited, 0, -1).astype(np.int32)
+      m += O.m_edges(g, lev)
+    out["breadth_first_order"] = {
+        "value": round(m / t / 1e9, 5), "unit": "GTEPS", "cores": 1,
+        "sample": f"{bfo_roots} of {len(roots)} roots, scipy breadth_first_order on M^T "
+                  f"(the reference's own oracle), {t:.1f}s"}
+    log(f"cpu breadth_first_order: {out['breadth_first_order']['value']} GTEPS over {bfo_roots} roots")
+  if "value" not in out and "breadth_first_order" in out:
+    out["value"] = out["breadth_first_order"]["value"]
+    out["sample"] = out["breadth_first_order"]["sample"]
+  out.update(host_info())
+  out["csl_simulator"] = csl_simulator_status()
+  return out
+
+
+def git_head():
+  try:
+    return subprocess.run(["git", "-C", HERE, "rev-parse", "--short=12", "HEAD"], capture_output=True,
+                          text=True, check=False, timeout=10).stdout.strip() or None
+  except (OSError, subprocess.SubprocessError):
+    return None
+
+
 def parse_args():
   p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
   p.add_argument("--gpus", type=int, default=1)
   p.add_argument("--steps", type=int, default=5)
   p.add_argument("--warmup", type=int, default=2)
-  p.add_argument("--scale", type=int, default=24)
-  p.add_argument("--edgefactor", type=int, default=16)
-  p.add_argument("--seed", type=int, default=3, help="graph seed")
-  p.add_argument("--root-seed", type=int, default=4)
-  p.add_argument("--nroots", type=int, default=64)
-  p.add_argument("--cpu-roots", type=int, default=1,
-                 help="roots timed for the CPU baseline (0 disables it)")
-  p.add_argument("--no-per-root", action="store_true", help="skip the one-root-at-a-time leg")
+  p.add_argument("--workload", choices=sorted(WORKLOADS), default="c3")
+  p.add_argument("--cpu-roots", type=int, default=None,
+                 help="roots timed by the SpMV CPU baseline (default: c3 1, c2 and c1 64)")
+  p.add_argument("--cpu-bfo-roots", type=int, default=None,
+                 help="roots timed by scipy breadth_first_order (default: c3 2, c2 and c1 64)")
+  p.add_argument("--no-per-root", action="store_true", help="c3: skip the one-root-at-a-time leg")
   p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                  help="also run one device-to-device copy of MIB MiB (a known-size "
                       "reference kernel for scaling HBM byte counters)")
   return p.parse_args()
+
+
+def per_root_leg(dg, roots, device_out, warm=2):
+  import torch
+  for s in roots[:warm]:
+    dg.bfs(int(s), device_out=device_out)
+  torch.cuda.synchronize()
+  res = []
+  tw = time.perf_counter()
+  for s in roots:
+    res.append(dg.bfs(int(s), device_out=device_out))
+  torch.cuda.synchronize()
+  return res, time.perf_counter() - tw
+
+
+def per_root_summary(n, res, wall):
+  tms = np.array([r.stats.total_ms for r in res])
+  m = np.array([r.stats.m_edges for r in res], dtype=np.float64)
+  hm = harmonic_mean(m / (tms / 1e3) / 1e9)
+  roof = per_root_roofline(n, res)
+  traffic, src = traffic_from_profile("bfs_pull_s24_traffic.json", "dominant_level_hbm_bytes")
+  if n == 1 << 24:
+    roof["traffic"], roof["traffic_source"] = traffic, src
+  return {"hmean_gteps": round(hm, 3), "aggregate_gteps": round(m.sum() / wall / 1e9, 3),
+          "ms_mean": round(float(tms.mean()), 4), "wall_s": round(wall, 4), "roofline": roof}, m, tms
 
 
 def main():
@@ -76,15 +253,26 @@ def main():
   import torch
   from ceregraphs_amd.engine import DeviceGraph
 
+  W = WORKLOADS[args.workload]
   t0 = time.perf_counter()
-  dg = DeviceGraph.from_rmat(args.scale, args.edgefactor, args.seed)
+  dg = DeviceGraph.from_rmat(W["scale"], W["edgefactor"], W["seed"])
+  if W.get("balance_grid"):
+    from ceregraphs_amd import batch as B
+    ip, ii, op, oi = dg.download()
+    dg.close()
+    part, _ = B.balance_part((len(ip) - 1, ip, ii, op, oi), W["balance_grid"], symmetric=True)
+    dg = DeviceGraph.from_csr(part[0], part[3], part[4], symmetric=True)
   gen_s = time.perf_counter() - t0
   n = dg.n
-  log(f"graph s{args.scale} ef{args.edgefactor} seed{args.seed}: n={n} nnz={dg.nnz} "
-      f"(generated on GPU in {gen_s:.2f}s)")
+  log(f"workload {args.workload}: n={n} nnz={dg.nnz} (built in {gen_s:.2f}s)")
   deg = dg.out_degree()
   cands = np.nonzero(deg > 0)[0]  # RMAT output has no self-loops
-  roots = np.random.default_rng(args.root_seed).choice(cands, size=args.nroots, replace=False)
+  roots = np.random.default_rng(W["root_seed"]).choice(cands, size=W["nroots"], replace=False)
+  if "source" in W:
+    roots = np.concatenate([[W["source"]], roots[roots != W["source"]][:W["nroots"] - 1]])
+  spmv_roots = args.cpu_roots if args.cpu_roots is not None else (1 if args.workload == "c3" else 64)
+  bfo_roots = args.cpu_bfo_roots if args.cpu_bfo_roots is not None else (2 if args.workload == "c3" else 64)
+  head = git_head()
 
   if args.calibrate_copy:
     src = torch.ones(args.calibrate_copy << 18, dtype=torch.int32, device="cuda")
@@ -93,11 +281,61 @@ def main():
     torch.cuda.synchronize()
     log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
     del src, dst
-  par = torch.empty(args.nroots * n, dtype=torch.int32, device="cuda")
-  lev = torch.empty(args.nroots * n, dtype=torch.int16, device="cuda")
+
+  if args.workload in ("c1", "c2"):
+    p1 = torch.empty(n, dtype=torch.int32, device="cuda")
+    l1 = torch.empty(n, dtype=torch.int32, device="cuda")
+    if args.workload == "c2":
+      for _ in range(args.warmup):
+        per_root_leg(dg, roots, (p1, l1), warm=0)
+      res, wall = [], 0.0
+      for _ in range(args.steps):
+        r, w = per_root_leg(dg, roots, (p1, l1), warm=0)
+        res, wall = res + r, wall + w
+      summary, m, tms = per_root_summary(n, res, wall)
+      value = summary["hmean_gteps"]
+      ms_step = wall / len(res) * 1e3
+      metric = "harmonic-mean GTEPS, RMAT s21 ef16, 64 roots, one search per launch"
+      cfg = {"workload": "bfs-rmat-s21-ef16-per-root", "roots": len(roots)}
+    else:
+      s = int(roots[0])
+      for _ in range(args.warmup):
+        dg.bfs(s, device_out=(p1, l1))
+      res, wall = per_root_leg(dg, [s] * args.steps, (p1, l1), warm=0)
+      summary, m, tms = per_root_summary(n, res, wall)
+      value = summary["hmean_gteps"]
+      ms_step = wall / len(res) * 1e3
+      metric = "GTEPS, RMAT s10 ef16 balanced 4x4, source 0"
+      cfg = {"workload": "bfs-rmat-s10-ef16-balanced4x4-src0", "source": s}
+    log(f"{args.workload}: {value:.3f} GTEPS hmean, {summary['ms_mean']:.4f} ms per search "
+        f"(device), {ms_step:.4f} ms wall")
+    lev_of = {}
+
+    def lanes_check(l):
+      if l not in lev_of:
+        dg.bfs(int(roots[l]), device_out=(p1, l1))
+        lev_of[l] = l1.cpu().numpy()
+      return lev_of[l]
+    cpu = cpu_baseline(dg, roots, spmv_roots, bfo_roots, lanes_check)
+    out = {"metric": metric, "value": value, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+           "data": "synthetic (RMAT generated on the GPU, edge-identical to gen_rmat.py)",
+           "config": dict(cfg, scale=W["scale"], edgefactor=W["edgefactor"],
+                          graph_seed=W["seed"], root_seed=W["root_seed"], n=n, nnz=dg.nnz),
+           "roofline": summary["roofline"], "cpu_baseline": cpu, "per_root": summary,
+           "head": head}
+    print(json.dumps(out), flush=True)
+    dg.close()
+    return
+
+  plane = len(roots) * n
+  par = torch.empty(plane, dtype=torch.int32, device="cuda")
+  lev = torch.empty(plane, dtype=torch.int16, device="cuda")
   # Timed passes decide every level on the device (the round kernels are
   # replayed from a captured HIP graph); one more pass with per-round host
-  # polls and events gives the per-round breakdown and the pull roofline.
+  # polls and events gives the per-round breakdown and the pull roofline,
+  # and must leave exactly the outputs of the timed passes.
   for i in range(args.warmup):
     r = dg.msbfs(roots, device_out=(par, lev), capture=True)
     log(f"warmup {i}: {r.stats.total_ms:.3f} ms, rounds={r.stats.rounds_completed}")
@@ -112,8 +350,12 @@ def main():
   wall = time.perf_counter() - t1
   ms_per_step = wall / args.steps * 1e3
   m = results[-1].m_lanes.astype(np.float64)
+  timed_par, timed_lev = par.clone(), lev.clone()
   last = dg.msbfs(roots, device_out=(par, lev))
-  if not np.array_equal(last.m_lanes, results[-1].m_lanes):
+  same = bool(torch.equal(timed_par, par) and torch.equal(timed_lev, lev)
+              and np.array_equal(last.m_lanes, results[-1].m_lanes))
+  del timed_par, timed_lev
+  if not same:
     log("WARNING: the instrumented pass differs from the timed passes")
   log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step wall, "
       f"{np.mean(step_ms):.3f} ms/step events; instrumented pass {last.stats.total_ms:.3f} ms, "
@@ -142,14 +384,10 @@ def main():
   tpull = float(st.pull_ms[pr].sum()) / 1e3
   npull = len(pr)
   achieved = byts / tpull / 1e9 if tpull > 0 else 0.0
-  traffic = None
-  prof = os.path.join(HERE, "profiles", "msbfs_pull_s24_traffic.json")
-  if os.path.exists(prof):
-    with open(prof, encoding="utf-8") as f:
-      traffic = json.load(f).get("hbm_bytes_per_launch")
+  traffic, tsrc = traffic_from_profile("msbfs_pull_s24_traffic.json")
   roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
               "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-              "kernel": "ms_pull_segs+ms_pull", "launches": npull,
+              "traffic_source": tsrc, "kernel": "ms_pull_segs+ms_pull", "launches": npull,
               "avg_launch_ms": round(tpull * 1e3 / max(npull, 1), 4),
               "algorithmic_bytes_per_launch": int(byts / max(npull, 1))}
   log(f"pull rounds: {npull}, {tpull*1e3:.3f} ms, {byts/1e9:.2f} GB algorithmic -> "
@@ -159,57 +397,30 @@ def main():
   if not args.no_per_root:
     p1 = torch.empty(n, dtype=torch.int32, device="cuda")
     l1 = torch.empty(n, dtype=torch.int32, device="cuda")
-    for s in roots[:2]:
-      dg.bfs(int(s), device_out=(p1, l1))
-    tms, ms_root = [], []
-    torch.cuda.synchronize()
-    tw = time.perf_counter()
-    for s in roots:
-      rr = dg.bfs(int(s), device_out=(p1, l1))
-      tms.append(rr.stats.total_ms)
-      ms_root.append(rr.stats.m_edges)
-    torch.cuda.synchronize()
-    wall_roots = time.perf_counter() - tw
-    tms = np.array(tms)
-    ms_root = np.array(ms_root, dtype=np.float64)
+    res, wall_roots = per_root_leg(dg, roots, (p1, l1))
+    summary, ms_root, tms = per_root_summary(n, res, wall_roots)
     if not np.array_equal(ms_root, m):
       log("WARNING: per-root m differs from the multi-source lanes")
-    per_root_hmean = harmonic_mean(ms_root / (tms / 1e3) / 1e9)
-    per_root_aggregate = ms_root.sum() / wall_roots / 1e9
     extra.update({
-        "per_root_hmean_gteps": round(per_root_hmean, 3),
-        "per_root_aggregate_gteps": round(per_root_aggregate, 3),
-        "per_root_ms_mean": round(float(tms.mean()), 4),
-        "per_root_wall_s": round(wall_roots, 4),
-        "batched_vs_per_root_aggregate": round(batched_aggregate / per_root_aggregate, 2),
+        "per_root_hmean_gteps": summary["hmean_gteps"],
+        "per_root_aggregate_gteps": summary["aggregate_gteps"],
+        "per_root_ms_mean": summary["ms_mean"],
+        "per_root_wall_s": summary["wall_s"],
+        "batched_vs_per_root_aggregate": round(batched_aggregate / summary["aggregate_gteps"], 2),
+        "per_root": summary,
     })
-    log(f"per-root: hmean {per_root_hmean:.2f} GTEPS, mean {tms.mean():.3f} ms/root, "
-        f"aggregate {per_root_aggregate:.2f} GTEPS over {wall_roots:.3f}s")
+    log(f"per-root: hmean {summary['hmean_gteps']:.2f} GTEPS, mean {summary['ms_mean']:.3f} ms/root, "
+        f"aggregate {summary['aggregate_gteps']:.2f} GTEPS; dominant level "
+        f"{summary['roofline']['achieved']:.0f} GB/s")
 
   cpu = None
-  if args.cpu_roots > 0:
-    from oracle import oracle as O
-    ip, ii, op, oi = dg.download()
-    g = O.Graph(n, ip, ii, op, oi, True)
-    M = g.matrix().astype(np.int32)
-    cpu_t, cpu_m = 0.0, 0
-    for l in range(args.cpu_roots):
-      s = int(roots[l])
-      tc = time.perf_counter()
-      lev_cpu, _, _, _ = O.spmv_bfs(g, s, matrix=M)
-      cpu_t += time.perf_counter() - tc
-      cpu_m += O.m_edges(g, lev_cpu)
-      lane = lev[l * n:(l + 1) * n].cpu().numpy()
-      if not np.array_equal(lane, lev_cpu.astype(np.int16)):
-        log(f"WARNING: CPU levels differ from lane {l}")
-    cpu = {"value": round(cpu_m / cpu_t / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-           "sample": f"{args.cpu_roots} of {args.nroots} roots, scipy boolean-SpMV BFS "
-                     f"(oracle.spmv_bfs) over the same s{args.scale} graph, "
-                     f"{cpu_t:.1f}s on {os.cpu_count()} visible host cores (1 used)"}
-    log(f"cpu baseline: {cpu['value']} GTEPS in {cpu_t:.1f}s")
+  if spmv_roots or bfo_roots:
+    cpu = cpu_baseline(dg, roots, spmv_roots, bfo_roots,
+                       lambda l: lev[l * n:(l + 1) * n].cpu().numpy().astype(np.int32))
+    log(f"cpu baseline: {cpu['value']} GTEPS")
 
   out = {
-      "metric": "harmonic-mean GTEPS, RMAT s24 ef16, 64 roots: batched MS-BFS vs per-root",
+      "metric": METRIC_C3,
       "value": round(batched_hmean, 3),
       "unit": "GTEPS",
       "n_gpus": 1,
@@ -221,16 +432,18 @@ def main():
       "vs_baseline": None,
       "dtype": "u64",
       "data": "synthetic (RMAT generated on the GPU, edge-identical to gen_rmat.py)",
-      "config": {"workload": f"msbfs-rmat-s{args.scale}-ef{args.edgefactor}",
-                 "scale": args.scale, "edgefactor": args.edgefactor, "graph_seed": args.seed,
-                 "root_seed": args.root_seed, "nroots": int(len(roots)), "n": n,
+      "config": {"workload": f"msbfs-rmat-s{W['scale']}-ef{W['edgefactor']}",
+                 "scale": W["scale"], "edgefactor": W["edgefactor"], "graph_seed": W["seed"],
+                 "root_seed": W["root_seed"], "nroots": int(len(roots)), "n": n,
                  "nnz": dg.nnz},
       "batched_aggregate_gteps": round(batched_aggregate, 3),
       "batched_lane_hmean_gteps": round(lane_hmean, 3),
+      "timed_outputs_equal_instrumented": same,
       "msbfs_rounds": int(st.rounds_completed),
       "graph_gen_s": round(gen_s, 3),
       "roofline": roofline,
       "cpu_baseline": cpu,
+      "head": head,
   }
   out.update(extra)
   print(json.dumps(out), flush=True)

@@ -34,9 +34,13 @@ constexpr int kHistCap = 4096;        // rounds recorded in the history arrays
 constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/above this
 constexpr int kBigLds = 4096;         // push: big-list prefix entries staged per pass
 constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this
-constexpr int64_t kPullWaveDegree = 256;  // pull: wavefront scan at/above this
-constexpr int kPullUnroll = 4;         // pull: row entries a lane tests per step
-constexpr int32_t kPullGrab = 4;       // pull: chunks per grab from the cursors
+constexpr int kPullUnroll = 4;         // pull: entries of each row the head probe tests
+constexpr int kPassSteps = 2;          // pull: 64-entry steps per tail pass
+constexpr int kPass = kPassSteps * kWave;  // pull: tail entries per pass
+#ifndef CG_PULL_GRAB
+#define CG_PULL_GRAB 16
+#endif
+constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the cursors
 constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
 
 struct Ctrl {
@@ -54,6 +58,7 @@ struct Ctrl {
   long long rounds;
   int dir;   // direction of the current round: 0 push, 1 pull
   int light; // the current round's pull splits its chunks statically
+  int listed; // the current round's frontier is in its queue (a pull leaves only the bitmap)
   int done;
   int status;
   unsigned long long t0;
@@ -209,6 +214,7 @@ struct LevelArgs {
   unsigned long long *cursor;  // pull chunk cursors, zero between rounds
   int64_t n;
   int64_t nwords;
+  int symmetric;  // in view == out view: a vertex's out-degree is its in-row length
 };
 
 // ---------------------------------------------------------------------------
@@ -249,6 +255,7 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, BfsParams P, long long t
   st_ctrl(&c->prev_nf, 1ll);
   st_ctrl(&c->seen, 1ll);
   st_ctrl(&c->light, 0);
+  st_ctrl(&c->listed, 1);
   st_ctrl(&c->rounds, 0ll);
   // round 0 is push (bool_pe.csl:714-719) unless every round pulls
   st_ctrl(&c->dir, P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0);
@@ -375,6 +382,28 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
   if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
+// Lists the frontier bitmap of round d in its queue when a push round
+// follows a pull round (a pull leaves only the bitmap).  Words are read in
+// order and appended through a block-staged queue, one global atomic per
+// block; the frontier is small when a pull hands over to a push.
+__global__ void __launch_bounds__(kBlock) bfs_compact(LevelArgs A, int d) {
+  Ctrl *c = A.ctrl;
+  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0 || ld_ctrl(&c->listed)) return;
+  const uint32_t *fcur = A.fb[d % 3];
+  __shared__ int32_t lq[kLdsQueue];
+  __shared__ unsigned lq_n;
+  __shared__ unsigned long long lq_base;
+  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kLdsQueue, &c->qtail[d & 1], A.q[d & 1]};
+  sq.init();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nround = ((A.n + stride - 1) / stride) * stride;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nround; v += stride) {
+    const bool on = v < A.n && ((fcur[v >> 5] >> (v & 31)) & 1u);
+    sq.push(on, (int32_t)v);
+  }
+  sq.flush();
+}
+
 // Degree prefix of the push round's big list (one block): bigpre[j] = sum of
 // the degrees of bigq[0..j), bigpre[nb] = big_edges.
 __global__ void __launch_bounds__(1024) bfs_push_scan(LevelArgs A) {
@@ -479,107 +508,188 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
 // ---------------------------------------------------------------------------
 // Pull round
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
+// What a chunk needs before its rows can be scanned: its two visited words
+// and, in a dense round, the row offsets of all 64 vertices.  Loaded one
+// chunk ahead, so those loads are in flight while the chunk before is being
+// scanned and a chunk's scan starts with its first row entries.
+struct PullPre {
+  int32_t ch;       // chunk, -1 for none
+  uint32_t vlo, vhi;
+  int64_t beg, end;
+};
+
+// A dense round (most vertices unvisited) loads every lane's offsets with
+// the visited words, so the two loads overlap; a light round (ctrl->light)
+// loads offsets only for unvisited vertices, once their bits are known,
+// since almost every vertex is visited there.
+__device__ __forceinline__ void pull_prefetch(const LevelArgs &A, int32_t ch, bool light,
+                                              PullPre &p) {
+  p.ch = ch;
+  if (ch < 0) return;
+  const int64_t w0 = (int64_t)ch * 2;
+  p.vlo = A.visited[w0];
+  p.vhi = A.visited[w0 + 1];
+  if (!light) {
+    const int64_t v = (int64_t)ch * kWave + lane_id();
+    const int64_t vv = v < A.n ? v : A.n - 1;
+    p.beg = A.in_ptr[vv];
+    p.end = A.in_ptr[vv + 1];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock, 8) bfs_pull_round(LevelArgs A, int d) {
   Ctrl *c = A.ctrl;
   if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 1) return;
   __shared__ unsigned long long red[kWavesPerBlock];
   const uint32_t *fcur = A.fb[d % 3];
   uint32_t *fnext = A.fb[(d + 1) % 3];
   uint32_t *fclear = A.fb[(d + 2) % 3];
-  int32_t *QN = A.q[(d + 1) & 1];
-  unsigned long long *gtail = &c->qtail[(d + 1) & 1];
-  __shared__ int32_t lq[kLdsQueue];
-  __shared__ unsigned lq_n;
-  __shared__ unsigned long long lq_base;
-  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kLdsQueue, gtail, QN};
-  sq.init();
   int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t gstride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t w = gtid; w < A.nwords; w += gstride) fclear[w] = 0;
 
   const int lane = lane_id();
+  // row map of the current tail pass, [step][position], one per wavefront
+  __shared__ int32_t rmap[kBlock / kWave][kPassSteps][kWave];
+  int32_t (*rm)[kWave] = rmap[threadIdx.x / kWave];
   const int64_t nchunks = A.nwords / 2;
-  unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0;
+  const bool light = ld_ctrl(&c->light) != 0;
+  const bool sym = A.symmetric != 0;
+  unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0, found = 0;
   // One chunk of 64 vertices (wavefront-uniform).
-  auto chunk = [&](int64_t ch) {
+  auto chunk = [&](const PullPre &p) {
+    const int64_t ch = p.ch;
     const int64_t w0 = ch * 2;
-    uint32_t vlo = A.visited[w0], vhi = A.visited[w0 + 1];
+    const uint32_t vlo = p.vlo, vhi = p.vhi;
     const int64_t v = ch * kWave + lane;
     uint32_t myword = (lane < 32) ? vlo : vhi;
     bool todo = v < A.n && !((myword >> (lane & 31)) & 1u);
     if (!__any(todo)) return;
-    int64_t beg = todo ? A.in_ptr[v] : 0, end = todo ? A.in_ptr[v + 1] : 0;
-    bool longrow = todo && (end - beg) >= kPullWaveDegree;
+    int64_t beg, end;
+    if (light) {
+      beg = todo ? A.in_ptr[v] : 0;
+      end = todo ? A.in_ptr[v + 1] : 0;
+    } else {
+      beg = todo ? p.beg : 0;
+      end = todo ? p.end : 0;
+    }
     bool hit = false;
     int32_t par = -1;
     if (todo) rows++;
-    if (todo && !longrow) {
-      // kPullUnroll entries per step: their index loads, then their
-      // frontier-bit loads, go out together; the first hit in row order is
-      // still the lowest id.  Lanes past the row end reread its first entry.
-      for (int64_t e0 = beg; e0 < end && !hit; e0 += kPullUnroll) {
-        int32_t u[kPullUnroll];
-        uint32_t fw[kPullUnroll];
+    // Head probe: every row's first kPullUnroll entries at once, one lane
+    // per row (their index loads, then their frontier-bit loads, in flight
+    // together); the first hit in row order is the lowest id.  Most rows of
+    // a dense level end here.  Lanes past the row end reread its first entry.
+    const int64_t hend = end - beg < kPullUnroll ? end : beg + kPullUnroll;
+    if (todo && beg < end) {
+      int32_t u[kPullUnroll];
+      uint32_t fw[kPullUnroll];
 #pragma unroll
-        for (int k = 0; k < kPullUnroll; k++) u[k] = A.in_idx[e0 + k < end ? e0 + k : e0];
+      for (int k = 0; k < kPullUnroll; k++) u[k] = A.in_idx[beg + k < hend ? beg + k : beg];
 #pragma unroll
-        for (int k = 0; k < kPullUnroll; k++) fw[k] = fcur[u[k] >> 5];
+      for (int k = 0; k < kPullUnroll; k++) fw[k] = fcur[u[k] >> 5];
 #pragma unroll
-        for (int k = 0; k < kPullUnroll; k++) {
-          if (!hit && e0 + k < end) {
-            scanned++;
-            if ((fw[k] >> (u[k] & 31)) & 1u) {
-              hit = true;
-              par = u[k];
-            }
+      for (int k = 0; k < kPullUnroll; k++) {
+        if (!hit && beg + k < hend) {
+          scanned++;
+          if ((fw[k] >> (u[k] & 31)) & 1u) {
+            hit = true;
+            par = u[k];
           }
         }
       }
     }
-    // Long in-neighbour lists: the wave scans 64 entries at a time; the
-    // lowest lane with a frontier hit holds the lowest id of the batch.
-    uint64_t lr = ballot64(longrow);
-    while (lr) {
-      int j = __ffsll((unsigned long long)lr) - 1;
-      lr &= lr - 1;
-      int64_t bj = __shfl(beg, j), ej = __shfl(end, j);
-      int32_t found = -1;
-      for (int64_t e0 = bj; e0 < ej; e0 += kWave) {
-        int64_t e = e0 + lane;
-        bool h = false;
-        int32_t u = 0;
-        if (e < ej) {
-          u = A.in_idx[e];
-          h = (fcur[u >> 5] >> (u & 31)) & 1u;
-        }
-        uint64_t hm = ballot64(h);
-        if (hm) {
-          int k = __ffsll((unsigned long long)hm) - 1;
-          found = __shfl(u, k);
-          if (lane == 0) scanned += (unsigned long long)(e0 - bj + k + 1);
-          break;
-        }
-        if (lane == 0) scanned += (unsigned long long)((ej - e0) < kWave ? (ej - e0) : kWave);
+    // Tails of the rows the probe left open, laid end to end and scanned
+    // kPass entries per pass by the whole wavefront, so no lane idles while
+    // another walks a long row.  Each pass packs what is still open (rows
+    // settled in the last pass drop out), maps every position to its row
+    // (row owners mark where their stretch starts; a prefix max fills the
+    // rest), loads ids and frontier bits branch-free, and each row's owner
+    // takes the first hit of its stretch from the pass's ballot masks: the
+    // stretch is the row's next entries in order, so that hit is its
+    // lowest-id parent if it has one there.
+    int64_t pos = hend;                                      // my row's next entry
+    int64_t rem = (todo && !hit) ? end - hend : 0;           // its entries left
+    while (__any(rem > 0)) {
+      const int32_t len = (int32_t)(rem < kPass ? rem : kPass);  // what a pass can take
+      const int32_t incl = wave_scan_add(len);
+      const int32_t vstart = incl - len;
+      const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+#pragma unroll
+      for (int k = 0; k < kPassSteps; k++) rm[k][lane] = 0;
+      wave_sync();
+      if (len > 0 && vstart < kPass) rm[vstart / kWave][vstart % kWave] = lane + 1;
+      wave_sync();
+      int32_t u[kPassSteps];
+      bool ok[kPassSteps];
+      int carry = 0;
+#pragma unroll
+      for (int k = 0; k < kPassSteps; k++) {
+        const int r1 = max(wave_scan_max(rm[k][lane]), carry);  // row + 1 (0: none yet)
+        carry = __builtin_amdgcn_readlane(r1, kWave - 1);
+        const int32_t ve = k * kWave + lane;
+        ok[k] = ve < total;
+        const int r = ok[k] ? r1 - 1 : 0;
+        const int32_t vs = __shfl(vstart, r);
+        const int64_t ps = __shfl(pos, r);
+        u[k] = A.in_idx[ok[k] ? ps + (ve - vs) : 0];
       }
-      if (lane == j && found >= 0) {
-        hit = true;
-        par = found;
+      uint64_t H[kPassSteps];
+#pragma unroll
+      for (int k = 0; k < kPassSteps; k++) {
+        const uint32_t w = fcur[u[k] >> 5];
+        H[k] = ballot64(ok[k] && ((w >> (u[k] & 31)) & 1u));
+      }
+      // the owner's stretch in this pass: positions [vstart, vstop)
+      const bool mine = len > 0 && vstart < kPass;
+      const int32_t vstop = vstart + len < kPass ? vstart + len : kPass;
+      int32_t first = -1;  // the stretch's first hit position
+#pragma unroll
+      for (int k = 0; k < kPassSteps; k++) {
+        const int32_t lo = max(vstart, k * kWave) - k * kWave, hi = min(vstop, (k + 1) * kWave) - k * kWave;
+        if (first < 0 && mine && lo < hi) {
+          const uint64_t inside = (hi == kWave ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
+          const uint64_t h = H[k] & inside;
+          if (h) first = k * kWave + __ffsll((unsigned long long)h) - 1;
+        }
+      }
+      // the id at the hit position, read from the lane that loaded it
+      const int src = first >= 0 ? first % kWave : lane;
+      int32_t hid = 0;
+#pragma unroll
+      for (int k = 0; k < kPassSteps; k++) {
+        const int32_t x = __shfl(u[k], src);
+        if (first >= 0 && first / kWave == k) hid = x;
+      }
+      if (mine) {
+        if (first >= 0) {
+          hit = true;
+          par = hid;
+          scanned += (unsigned long long)(first - vstart + 1);
+          rem = 0;
+        } else {
+          scanned += (unsigned long long)(vstop - vstart);
+          pos += vstop - vstart;
+          rem -= vstop - vstart;
+        }
       }
     }
     if (hit) {
       A.parents[v] = (uint32_t)par;
       A.levels[v] = d + 1;
-      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
+      mf += (unsigned long long)(sym ? end - beg : A.out_ptr[v + 1] - A.out_ptr[v]);
       in_sum += (unsigned long long)(end - beg);
+      found++;
     }
+    // The new frontier is the bitmap alone: a pull round is followed by a
+    // pull far more often than by a push, which lists the bitmap first
+    // (bfs_compact).  Appending every found vertex to a queue here made the
+    // dense level wait on one shared tail counter.
     uint64_t hm = ballot64(hit);
-    if (hm) {
-      if (lane == 0) {
-        uint32_t lo = (uint32_t)hm, hi = (uint32_t)(hm >> 32);
-        if (lo) { fnext[w0] = lo; A.visited[w0] = vlo | lo; }
-        if (hi) { fnext[w0 + 1] = hi; A.visited[w0 + 1] = vhi | hi; }
-      }
-      sq.push(hit, (int32_t)v);
+    if (hm && lane == 0) {
+      uint32_t lo = (uint32_t)hm, hi = (uint32_t)(hm >> 32);
+      if (lo) { fnext[w0] = lo; A.visited[w0] = vlo | lo; }
+      if (hi) { fnext[w0 + 1] = hi; A.visited[w0 + 1] = vhi | hi; }
     }
   };
   // Chunks come from the cursors: their costs differ by orders of magnitude
@@ -587,11 +697,21 @@ __global__ void __launch_bounds__(kBlock) bfs_pull_round(LevelArgs A, int d) {
   // static split over the wavefronts leaves unbalanced.  Once few vertices
   // with in-edges are unreached, nearly every chunk is visited and costs
   // less than the atomic that would hand it out: a static split then.
-  ChunkFeed feed(A.cursor, (int32_t)nchunks, kPullGrab, ld_ctrl(&c->light) != 0);
+  // The chunk after the current one is prefetched before the current one is
+  // scanned.
+  ChunkFeed feed(A.cursor, (int32_t)nchunks, kPullGrab, light);
   int32_t cb = 0, ce = 0;
-  while (cb < ce || feed.next(cb, ce)) chunk(cb++);
-  sq.flush();
+  auto next_chunk = [&]() -> int32_t { return (cb < ce || feed.next(cb, ce)) ? cb++ : -1; };
+  PullPre cur, nxt;
+  pull_prefetch(A, next_chunk(), light, cur);
+  while (cur.ch >= 0) {
+    pull_prefetch(A, next_chunk(), light, nxt);
+    chunk(cur);
+    cur = nxt;
+  }
   unsigned long long s;
+  s = block_sum(found, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->nf, s);
   s = block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
   s = block_sum(in_sum, red);
@@ -611,7 +731,8 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
   if (ld_ctrl(&c->done)) return;
   const int nxt = (d + 1) & 1;
   const int cur = ld_ctrl(&c->dir);
-  const long long nf = (long long)ld_ctrl(&c->qtail[nxt]);
+  // a push round listed its new vertices, a pull round counted them
+  const long long nf = cur == 0 ? (long long)ld_ctrl(&c->qtail[nxt]) : (long long)ld_ctrl(&c->nf);
   const long long mf = (long long)ld_ctrl(&c->mf);
   const long long mu = ld_ctrl(&c->mu) - (long long)ld_ctrl(&c->in_sum);
   unsigned long long now = wall_clock64();
@@ -652,6 +773,7 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
       }
   }
   st_ctrl(&c->prev_nf, nf);
+  st_ctrl(&c->listed, cur == 0 ? 1 : 0);
   // The next pull visits every chunk; once fewer vertices with in-edges are
   // unreached than there are chunks, most chunks have nothing to do.
   const long long seen = ld_ctrl(&c->seen) + nf;
@@ -743,6 +865,7 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
   A.cursor = ws.cursor;
   A.n = g->n;
   A.nwords = ws.nwords;
+  A.symmetric = g->in_ptr == g->out_ptr ? 1 : 0;
 
   hipStream_t st = g->stream;
   const int blocks = grid_blocks(g->device, o.grid_mult);
@@ -777,6 +900,7 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
         rev[nrev++] = rb;
         hipEventRecord(ra, st);
       }
+      bfs_compact<<<blocks, kBlock, 0, st>>>(A, (int)d);
       bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
       bfs_push_scan<<<1, 1024, 0, st>>>(A);
       bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);

@@ -273,6 +273,50 @@ struct StagedQueue {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Cross-lane scans on DPP (VALU, no LDS round trip)
+// ---------------------------------------------------------------------------
+// GFX9 DPP controls used by the scans.
+constexpr int kDppRowShr = 0x110;     // + k: lane j reads lane j-k of its 16-lane row
+constexpr int kDppWaveShr1 = 0x138;   // lane j reads lane j-1
+constexpr int kDppRowBcast15 = 0x142; // rows 1, 3 (row mask 0xA) read lane 15 of the row before
+constexpr int kDppRowBcast31 = 0x143; // rows 2, 3 (row mask 0xC) read lane 31
+
+
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ int32_t dpp32(int32_t x) {
+  return __builtin_amdgcn_update_dpp(0, x, kCtrl, kRowMask, 0xF, true);
+}
+
+// Inclusive prefix sum / prefix max of non-negative values across the
+// wavefront (0 is the identity of both on non-negative values).
+__device__ __forceinline__ int32_t wave_scan_add(int32_t x) {
+  x += dpp32<kDppRowShr + 1>(x);
+  x += dpp32<kDppRowShr + 2>(x);
+  x += dpp32<kDppRowShr + 4>(x);
+  x += dpp32<kDppRowShr + 8>(x);
+  x += dpp32<kDppRowBcast15, 0xA>(x);
+  x += dpp32<kDppRowBcast31, 0xC>(x);
+  return x;
+}
+__device__ __forceinline__ int32_t wave_scan_max(int32_t x) {
+  x = max(x, dpp32<kDppRowShr + 1>(x));
+  x = max(x, dpp32<kDppRowShr + 2>(x));
+  x = max(x, dpp32<kDppRowShr + 4>(x));
+  x = max(x, dpp32<kDppRowShr + 8>(x));
+  x = max(x, dpp32<kDppRowBcast15, 0xA>(x));
+  x = max(x, dpp32<kDppRowBcast31, 0xC>(x));
+  return x;
+}
+
+// Orders a wavefront's LDS writes before its later LDS reads (no block
+// barrier: the callers loop with wavefront-divergent trip counts).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sums a 64-bit value across the wavefront.
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);

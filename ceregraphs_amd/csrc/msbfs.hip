@@ -56,9 +56,18 @@ constexpr int kMsHist = 4096;
 constexpr int64_t kSeg = 256;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
-constexpr int kFlush = 16;          // lane bits per batch of the pull's output flush
+#ifndef CG_MS_FLUSH
+#define CG_MS_FLUSH 16
+#endif
+constexpr int kFlush = CG_MS_FLUSH;  // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
 constexpr int32_t kGrabSparse = 8;  // marks a pull known to be light: it splits its chunks statically
+// Waves per SIMD the chunk pull is compiled for (its register budget is
+// 512 / kMsPullWaves VGPRs).
+#ifndef CG_MS_PULL_WAVES
+#define CG_MS_PULL_WAVES 7
+#endif
+constexpr int kMsPullWaves = CG_MS_PULL_WAVES;
 // First-hit offsets within a wavefront-tier row (< kSeg) fit in a byte.
 using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
@@ -512,11 +521,6 @@ __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
       (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), kCtrl, kRowMask, 0xF, true);
   return ((uint64_t)hi << 32) | lo;
 }
-constexpr int kDppRowShr = 0x110;     // + k: lane j reads lane j-k of its 16-lane row
-constexpr int kDppWaveShr1 = 0x138;   // lane j reads lane j-1
-constexpr int kDppRowBcast15 = 0x142; // rows 1, 3 (row mask 0xA) read lane 15 of the row before
-constexpr int kDppRowBcast31 = 0x143; // rows 2, 3 (row mask 0xC) read lane 31
-
 // Segmented inclusive prefix-OR across the wavefront: lane j gets the OR of
 // x over lanes [head, j], where head <= j is the first lane of j's segment
 // (segments are runs of consecutive lanes).  Shifts within 16-lane rows,
@@ -545,32 +549,6 @@ __device__ __forceinline__ uint64_t lane_before(uint64_t x) { return dpp64<kDppW
 // Inclusive prefix-OR across the wavefront.
 __device__ __forceinline__ uint64_t wave_scan_or(uint64_t x) { return seg_scan_or(x, 0, lane_id()); }
 
-template <int kCtrl, int kRowMask = 0xF>
-__device__ __forceinline__ int32_t dpp32(int32_t x) {
-  return __builtin_amdgcn_update_dpp(0, x, kCtrl, kRowMask, 0xF, true);
-}
-
-// Inclusive prefix sum / prefix max of non-negative values across the
-// wavefront (0 is the identity of both on non-negative values).
-__device__ __forceinline__ int32_t wave_scan_add(int32_t x) {
-  x += dpp32<kDppRowShr + 1>(x);
-  x += dpp32<kDppRowShr + 2>(x);
-  x += dpp32<kDppRowShr + 4>(x);
-  x += dpp32<kDppRowShr + 8>(x);
-  x += dpp32<kDppRowBcast15, 0xA>(x);
-  x += dpp32<kDppRowBcast31, 0xC>(x);
-  return x;
-}
-__device__ __forceinline__ int32_t wave_scan_max(int32_t x) {
-  x = max(x, dpp32<kDppRowShr + 1>(x));
-  x = max(x, dpp32<kDppRowShr + 2>(x));
-  x = max(x, dpp32<kDppRowShr + 4>(x));
-  x = max(x, dpp32<kDppRowShr + 8>(x));
-  x = max(x, dpp32<kDppRowBcast15, 0xA>(x));
-  x = max(x, dpp32<kDppRowBcast31, 0xC>(x));
-  return x;
-}
-
 __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
                                                             unsigned long long *lds) {
   v = wave_sum_u64(v);
@@ -582,14 +560,6 @@ __device__ __forceinline__ unsigned long long ms_block_sum(unsigned long long v,
     for (int i = 0; i < (int)(blockDim.x / kWave); i++) s += lds[i];
   __syncthreads();
   return s;
-}
-
-// Orders a wavefront's LDS writes before its later LDS reads (no block
-// barrier: the callers loop with wavefront-divergent trip counts).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
@@ -881,7 +851,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
 // every bit is set and the extra load only adds latency) come from the
 // control block; tally_on keeps the per-lane tallies (per-lane policies).
 template <bool tally_on>
-__global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
+__global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
   if (!R.pm) return;
@@ -891,8 +861,14 @@ __global__ void __launch_bounds__(kMsBlock, 7) ms_pull(MsArgs A) {
   __shared__ unsigned long long red[kMsWaves];
   __shared__ LaneOutLds so;
   stage_out(A.out, d, so);
-  __shared__ unsigned long long tl[192];
-  tally_lds_init(tl);
+  // per-lane tallies (the per-lane direction policies only): 1.5 KB of
+  // LDS the default pull does without, so one more block fits per CU
+  unsigned long long *tl = nullptr;
+  if constexpr (tally_on) {
+    __shared__ unsigned long long tl_lds[192];
+    tl = tl_lds;
+    tally_lds_init(tl);
+  }
   LaneTally tally;
   const bool sym = A.symmetric != 0;
   const uint64_t *fr = A.fw[d & 1];
