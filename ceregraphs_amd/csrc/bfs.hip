@@ -34,8 +34,14 @@ constexpr int kHistCap = 4096;        // rounds recorded in the history arrays
 constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/above this
 constexpr int kBigLds = 4096;         // push: big-list prefix entries staged per pass
 constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this
-constexpr int kPullUnroll = 4;         // pull: entries of each row the head probe tests
-constexpr int kPassSteps = 2;          // pull: 64-entry steps per tail pass
+#ifndef CG_PULL_HEAD
+#define CG_PULL_HEAD 2
+#endif
+#ifndef CG_PULL_PASS_STEPS
+#define CG_PULL_PASS_STEPS 1
+#endif
+constexpr int kPullUnroll = CG_PULL_HEAD;  // pull: entries of each row the head probe tests
+constexpr int kPassSteps = CG_PULL_PASS_STEPS;  // pull: 64-entry steps per tail pass
 constexpr int kPass = kPassSteps * kWave;  // pull: tail entries per pass
 #ifndef CG_PULL_GRAB
 #define CG_PULL_GRAB 16

@@ -345,11 +345,18 @@ static int build_seglists(DeviceGraph *g, MsGraphData *md) {
 // Hub-first slots: vertices ordered by out-degree (descending, ties by id).
 // An in-neighbour u is gathered once per out-edge of u, so this packs the
 // most-gathered lane words into the first few MB, where they stay cached.
+#ifndef CG_SLOT_HUB_DEGREE
+#define CG_SLOT_HUB_DEGREE 0
+#endif
+// Vertices of out-degree below this keep id order behind the hubs (0: every
+// vertex is ordered by degree).
+constexpr int64_t kSlotHubDegree = CG_SLOT_HUB_DEGREE;
 __global__ void slot_keys_kernel(const int64_t *__restrict__ out_ptr, int64_t n,
                                  uint64_t *__restrict__ keys) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   int64_t deg = out_ptr[v + 1] - out_ptr[v];
+  if (deg < kSlotHubDegree) deg = 0;
   uint64_t rank = (uint64_t)(0x7FFFFFFFll - (deg < 0x7FFFFFFFll ? deg : 0x7FFFFFFFll));
   keys[v] = (rank << 32) | (uint64_t)v;
 }
