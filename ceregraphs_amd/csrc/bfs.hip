@@ -114,7 +114,7 @@ static constexpr size_t ws1_size(int64_t n, bool internal_out) {
          align_up(sizeof(int64_t) * (size_t)(n + 1)) +                      // bigpre
          (internal_out ? align_up(sizeof(int32_t) * (size_t)n) * 2 : 0) +   // levels, parents
          align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64) +
-         align_up(sizeof(unsigned long long) * kParts * kCursorStride);
+         align_up(sizeof(unsigned long long) * kMaxParts * kCursorStride);
 }
 
 // Lays the workspace of ws1_size(n, internal_out) bytes out from `base`.
@@ -164,7 +164,7 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
 // hipGetSymbolAddress, once per device, with that device current.
 constexpr int64_t kNoAllocMaxN = (int64_t)1 << 21;
 constexpr size_t kNoAllocBytes = ws1_size(kNoAllocMaxN, false);
-static_assert(kNoAllocBytes == 43140864, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
+static_assert(kNoAllocBytes == 43148032, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 constexpr int kMaxDevices = 64;

@@ -133,7 +133,12 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // A pull hands out its 64-vertex chunks dynamically from kParts cursors, one
 // per contiguous part of the vertex range, kept a cache line apart.
-constexpr int kParts = 8;
+#ifndef CG_PARTS
+#define CG_PARTS 8
+#endif
+constexpr int kParts = CG_PARTS;
+constexpr int kMaxParts = 64;  // cursor areas are sized for this many parts
+static_assert(kParts >= 1 && kParts <= kMaxParts, "kParts out of range");
 constexpr int kCursorStride = 16;
 
 // Zeroes the pull's chunk cursors for the next round (one thread).

@@ -62,6 +62,10 @@ constexpr int kUnroll = 2;          // pull steps whose loads are issued togethe
 constexpr int kFlush = CG_MS_FLUSH;  // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
 constexpr int32_t kGrabSparse = 8;  // marks a pull known to be light: it splits its chunks statically
+#ifndef CG_MS_DENSE_GRAB
+#define CG_MS_DENSE_GRAB 1
+#endif
+constexpr int32_t kGrabDense = CG_MS_DENSE_GRAB;  // chunks per grab of a dense pull
 // Waves per SIMD the chunk pull is compiled for (its register budget is
 // 512 / kMsPullWaves VGPRs).
 #ifndef CG_MS_PULL_WAVES
@@ -1521,7 +1525,7 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   // A pull rescans every row it opens from the start, so the next pull reads
   // at most what this one read: below one entry per vertex it is light, and
   // its chunks, most of which have nothing to scan, are split statically.
-  st_ctrl(&c->grab, cur_pm && pscanned < (long long)A.n ? kGrabSparse : 1);
+  st_ctrl(&c->grab, cur_pm && pscanned < (long long)A.n ? kGrabSparse : kGrabDense);
   // A pull part lists none of its new vertices: the next round's frontier is
   // listed only if this round did not pull, and a push part after a pull
   // rebuilds the list from an empty tail.
