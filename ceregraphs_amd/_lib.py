@@ -50,10 +50,13 @@ class Options(ctypes.Structure):
       ("alpha", ctypes.c_double),
       ("beta", ctypes.c_double),
       ("tau", ctypes.c_int64),
-      ("block", ctypes.c_int32),
+      ("pull_grab", ctypes.c_int32),
       ("grid_mult", ctypes.c_int32),
       ("max_levels", ctypes.c_int64),
       ("capture", ctypes.c_int32),
+      ("seg_grab", ctypes.c_int32),
+      ("push_wave_degree", ctypes.c_int32),
+      ("push_big_degree", ctypes.c_int32),
   ]
 
 

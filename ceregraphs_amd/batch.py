@@ -46,7 +46,9 @@ UNEVEN_ROOT_SEED = 18
 UNEVEN_ROOTS_PER_GRAPH = 32
 UNEVEN_BALANCE_GRID = 4  # P of the P x P balancing grid (0: generator ids)
 
-DEFAULT_CONFIG = {"direction": "beamer", "alpha": 14.0, "beta": 24.0, "grid_mult": 0}  # 0: by graph size
+# library defaults (grid_mult 0: by graph size)
+DEFAULT_CONFIG = {"direction": "beamer", "alpha": 14.0, "beta": 24.0, "grid_mult": 0,
+                  "pull_grab": 1, "seg_grab": 32}
 
 
 def union_csr(parts):
@@ -202,19 +204,47 @@ def _np(x):
   return x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
 
 
-def autotune(batch, roots, device_out=None, reps=3, space=None):
-  """Times every configuration of `space` on this batch (best of `reps`)
-  and returns (best config, [(config, ms)...] sorted by time)."""
-  space = space or {"direction": ["beamer", "beamer_lanes"], "alpha": [4.0, 14.0, 48.0],
-                    "beta": [12.0, 24.0, 96.0], "grid_mult": [2, 4, 8, 16]}
-  keys = list(space)
-  table = []
-  for values in itertools.product(*(space[k] for k in keys)):
-    cfg = dict(zip(keys, values))
-    best = min(batch.run(roots, device_out=device_out, **cfg).total_ms for _ in range(reps))
-    table.append((cfg, best))
-  table.sort(key=lambda t: t[1])
-  return table[0][0], table
+# The tuner's space: the direction policy and Beamer's alpha / beta, blocks
+# per CU of the grid-stride kernels, and the pull's work-distribution knobs
+# (chunks per grab in a dense round, hub-row segments per grab).  The
+# reference fixes its equivalents at compile time (layout_bool.csl:18-38,
+# the tau rule of run_bfs.py:239-243).
+TUNE_SPACE = {"direction": ["beamer", "beamer_lanes"], "alpha": [4.0, 14.0, 48.0],
+              "beta": [12.0, 24.0, 96.0], "grid_mult": [2, 4, 8, 16],
+              "pull_grab": [1, 2, 4], "seg_grab": [8, 32, 128]}
+
+
+def autotune(batch, roots, device_out=None, reps=3, space=None, passes=3):
+  """Coordinate descent over `space` (default TUNE_SPACE) from the
+  defaults: each knob in turn takes its best value with the others held,
+  until a pass changes nothing.  Each configuration is timed on the batch
+  itself (best of `reps` passes).  Returns (best config, [(config, ms)...]
+  sorted by time)."""
+  space = space or TUNE_SPACE
+  seen = {}
+
+  def cost(cfg):
+    key = tuple(sorted(cfg.items()))
+    if key not in seen:
+      seen[key] = min(batch.run(roots, device_out=device_out, **cfg).total_ms for _ in range(reps))
+    return seen[key]
+
+  start = dict(DEFAULT_CONFIG)
+  n = getattr(batch, "n", 0)
+  start["grid_mult"] = 2 if n < 2**19 else 4 if n < 2**23 else 8  # the library's choice for n
+  best = {k: (start[k] if start.get(k) in v else v[0]) for k, v in space.items()}
+  for _ in range(passes):
+    changed = False
+    for k, values in space.items():
+      scores = [(cost(dict(best, **{k: v})), v) for v in values]
+      t, v = min(scores, key=lambda x: x[0])
+      if v != best[k] and t < cost(best):
+        best[k] = v
+        changed = True
+    if not changed:
+      break
+  table = sorted(((dict(k), t) for k, t in seen.items()), key=lambda x: x[1])
+  return dict(best), table
 
 
 def _timed(batch, roots, cfg, device_out, steps):

@@ -31,9 +31,9 @@ namespace cg {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kHistCap = 4096;        // rounds recorded in the history arrays
-constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/above this
+constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/above this (default)
 constexpr int kBigLds = 4096;         // push: big-list prefix entries staged per pass
-constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this
+constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this (default)
 #ifndef CG_PULL_HEAD
 #define CG_PULL_HEAD 2
 #endif
@@ -46,7 +46,7 @@ constexpr int kPass = kPassSteps * kWave;  // pull: tail entries per pass
 #ifndef CG_PULL_GRAB
 #define CG_PULL_GRAB 16
 #endif
-constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the cursors
+constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the cursors (default)
 constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
 
 struct Ctrl {
@@ -221,6 +221,9 @@ struct LevelArgs {
   int64_t n;
   int64_t nwords;
   int symmetric;  // in view == out view: a vertex's out-degree is its in-row length
+  int64_t wave_degree;  // push: a wavefront expands vertices of out-degree >= this
+  int64_t big_degree;   // push: the whole grid expands vertices of out-degree >= this
+  int32_t pull_grab;    // pull: chunks per grab in a dense round
 };
 
 // ---------------------------------------------------------------------------
@@ -356,9 +359,9 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
     int64_t beg = valid ? A.out_ptr[u] : 0;
     int64_t end = valid ? A.out_ptr[u + 1] : 0;
     int64_t deg = end - beg;
-    bool big = valid && deg >= kBigDegree;
-    bool mid = valid && !big && deg >= kWaveDegree;
-    bool small = valid && deg < kWaveDegree;
+    bool big = valid && deg >= A.big_degree;
+    bool mid = valid && !big && deg >= A.wave_degree;
+    bool small = valid && deg < A.wave_degree;
     int64_t bslot = wave_append(&c->big_tail, big);
     if (big) A.bigq[bslot] = u;
     if (small) {
@@ -705,7 +708,7 @@ __global__ void __launch_bounds__(kBlock, 8) bfs_pull_round(LevelArgs A, int d) 
   // less than the atomic that would hand it out: a static split then.
   // The chunk after the current one is prefetched before the current one is
   // scanned.
-  ChunkFeed feed(A.cursor, (int32_t)nchunks, kPullGrab, light);
+  ChunkFeed feed(A.cursor, (int32_t)nchunks, A.pull_grab, light);
   int32_t cb = 0, ce = 0;
   auto next_chunk = [&]() -> int32_t { return (cb < ce || feed.next(cb, ce)) ? cb++ : -1; };
   PullPre cur, nxt;
@@ -841,6 +844,24 @@ void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o) {
   // from 2^23 (short rounds on small graphs pay for every extra block;
   // large graphs need the full grid).
   if (o->grid_mult <= 0) o->grid_mult = n < (1 << 19) ? 2 : n < (1 << 23) ? 4 : 8;
+  if (o->seg_grab <= 0) o->seg_grab = 32;
+  if (o->push_wave_degree <= 0) o->push_wave_degree = (int32_t)kWaveDegree;
+  if (o->push_big_degree <= 0)
+    o->push_big_degree = o->push_wave_degree > (int32_t)kBigDegree ? o->push_wave_degree : (int32_t)kBigDegree;
+}
+
+int check_options(const cgbfs_options *o) {
+  if (o->direction < 0 || o->direction > CGBFS_DIR_BEAMER_LANES)
+    return set_error(CGBFS_EINVAL, "unknown direction mode");
+  if (o->pull_grab < 0 || o->pull_grab > 1024) return set_error(CGBFS_EINVAL, "pull_grab must be 0 (default) or in [1, 1024]");
+  if (o->seg_grab < 1 || o->seg_grab > 1024) return set_error(CGBFS_EINVAL, "seg_grab must be in [1, 1024]");
+  if (o->grid_mult < 1 || o->grid_mult > 64) return set_error(CGBFS_EINVAL, "grid_mult must be in [1, 64]");
+  if (o->push_wave_degree < 1 || o->push_wave_degree > (1 << 30))
+    return set_error(CGBFS_EINVAL, "push_wave_degree must be in [1, 2^30]");
+  if (o->push_big_degree < o->push_wave_degree || o->push_big_degree > (1 << 30))
+    return set_error(CGBFS_EINVAL, "push_big_degree must be in [push_wave_degree, 2^30]");
+  if (!(o->alpha > 0) || !(o->beta > 0)) return set_error(CGBFS_EINVAL, "alpha and beta must be positive");
+  return CGBFS_OK;
 }
 
 }  // namespace cg
@@ -872,6 +893,9 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
   A.n = g->n;
   A.nwords = ws.nwords;
   A.symmetric = g->in_ptr == g->out_ptr ? 1 : 0;
+  A.wave_degree = o.push_wave_degree;
+  A.big_degree = o.push_big_degree;
+  A.pull_grab = o.pull_grab > 0 ? o.pull_grab : kPullGrab;
 
   hipStream_t st = g->stream;
   const int blocks = grid_blocks(g->device, o.grid_mult);
@@ -978,8 +1002,8 @@ static int bfs_args(cgbfs_graph *h, int64_t source, const void *parents, const v
   if (source < 0 || source >= h->g.n) return set_error(CGBFS_EINVAL, "source out of range");
   if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
   fill_options(opt, h->g.n, o);
-  if (o->direction < 0 || o->direction > CGBFS_DIR_BEAMER_LANES)
-    return set_error(CGBFS_EINVAL, "unknown direction mode");
+  const int rc = check_options(o);
+  if (rc) return rc;
   if (o->direction == CGBFS_DIR_BEAMER_LANES) o->direction = CGBFS_DIR_BEAMER;  // one lane
   return CGBFS_OK;
 }

@@ -61,7 +61,6 @@ constexpr int kUnroll = 2;          // pull steps whose loads are issued togethe
 #endif
 constexpr int kFlush = CG_MS_FLUSH;  // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
-constexpr int32_t kGrabSparse = 8;  // marks a pull known to be light: it splits its chunks statically
 #ifndef CG_MS_DENSE_GRAB
 #define CG_MS_DENSE_GRAB 1
 #endif
@@ -77,6 +76,7 @@ using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
 int grid_blocks(int device, int mult);
 void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o);
+int check_options(const cgbfs_options *o);
 
 // Round counters come in two parts: the push part (nf .. pairs) and the
 // pull part (pnf .. ppairs).  nf counts only vertices the pull did not
@@ -101,7 +101,7 @@ struct MsCtrl {
   int use_fb;                  // that round's pull reads the frontier bits
   int done;
   int status;
-  int grab;                    // chunks per grab of that round's pull
+  int grab;                    // chunks per grab of that round's pull; < 0: a static split
   int noin_left;               // outputs of vertices without in-edges not yet written
 };
 
@@ -144,6 +144,7 @@ struct MsParams {
   int mode;
   double alpha, beta;
   long long tau, max_levels, nnz;
+  int dense_grab;  // chunks per grab of a dense pull round
 };
 
 struct MsHist {
@@ -214,6 +215,7 @@ struct MsArgs {
   uint64_t act;      // mask of lanes in use
   int nroots;
   int symmetric;     // in view == out view
+  int seg_grab;      // hub-row segments per grab of ms_pull_segs
 };
 
 static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -803,7 +805,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->rounds, 0ll);
     st_ctrl(&c->done, 0);
     st_ctrl(&c->status, 0);
-    st_ctrl(&c->grab, 1);
+    st_ctrl(&c->grab, P.dense_grab);
     st_ctrl(&c->noin_left, 1);
     reset_ms_cursors(A.cursor);
   }
@@ -908,7 +910,7 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
   // Chunks come from the cursors: their costs differ by orders of magnitude
   // (row lengths, how soon rows are satisfied), which a static split of the
   // chunks over the wavefronts leaves unbalanced.
-  ChunkFeed feed(A.cursor, nch, R.grab, R.grab == kGrabSparse);
+  ChunkFeed feed(A.cursor, nch, R.grab > 0 ? R.grab : 1, R.grab < 0);
   int32_t cb = 0, ce = 0;
   while (cb < ce || feed.next(cb, ce)) {
     const int32_t ch = cb++;
@@ -1112,12 +1114,11 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
 // the finished first launch, or from this one) is already below the
 // segment's first id cannot improve and drops out; a stale read of a
 // candidate only keeps a lane in.  Segments come from their own cursors,
-// kSegGrab consecutive ones a grab: a segment stops at its first step that
+// seg_grab (32 by default) consecutive ones a grab: a segment stops at its first step that
 // finds every needed lane, so costs differ, and a run of one vertex's
 // segments in row order lets the later ones skip the lanes the earlier ones
 // found.  With fewer than kSegStatic segments per wavefront they are split
 // statically instead.
-constexpr int32_t kSegGrab = 32;
 constexpr int32_t kSegRun = 32;
 constexpr int32_t kSegStatic = 4;
 __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A, int late) {
@@ -1133,7 +1134,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A, int late) {
   const int32_t *segs = late ? A.segq_late : A.segq[0];
   const uint64_t *needs = late ? A.segneed_late : A.segneed;
   uint32_t scanned = 0;
-  ChunkFeed feed(A.cursor + (1 + late) * kParts * kCursorStride, nact, kSegGrab,
+  ChunkFeed feed(A.cursor + (1 + late) * kParts * kCursorStride, nact, A.seg_grab,
                  nact < kSegStatic * nwaves);
   int32_t gb, ge;
   // A vertex's segments are listed together in row order, so lanes this
@@ -1525,7 +1526,7 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   // A pull rescans every row it opens from the start, so the next pull reads
   // at most what this one read: below one entry per vertex it is light, and
   // its chunks, most of which have nothing to scan, are split statically.
-  st_ctrl(&c->grab, cur_pm && pscanned < (long long)A.n ? kGrabSparse : kGrabDense);
+  st_ctrl(&c->grab, cur_pm && pscanned < (long long)A.n ? -1 : P.dense_grab);
   // A pull part lists none of its new vertices: the next round's frontier is
   // listed only if this round did not pull, and a push part after a pull
   // rebuilds the list from an empty tail.
@@ -1714,7 +1715,7 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&c->use_fb, 0);
     st_ctrl(&c->done, 1);
     st_ctrl(&c->status, 0);
-    st_ctrl(&c->grab, 1);
+    st_ctrl(&c->grab, (int)kGrabDense);
     st_ctrl(&c->noin_left, 0);  // retiring lanes write their own
     reset_ms_cursors(A.cursor);
   }
@@ -1841,8 +1842,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   DeviceGraph *g = &h->g;
   cgbfs_options o;
   fill_options(opt, g->n, &o);
-  if (o.direction < 0 || o.direction > CGBFS_DIR_BEAMER_LANES)
-    return set_error(CGBFS_EINVAL, "unknown direction mode");
+  int rc0 = check_options(&o);
+  if (rc0) return rc0;
   if (o.max_levels > 32767) o.max_levels = 32767;  // int16 levels
   CG_HIP(hipSetDevice(g->device));
   MsWorkspace ws;
@@ -1911,6 +1912,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   A.nroots = nroots;
   A.act = nroots == 64 ? ~0ull : ((1ull << nroots) - 1ull);
   A.symmetric = g->in_ptr == g->out_ptr ? 1 : 0;
+  A.seg_grab = o.seg_grab;
 
   const int blocks = grid_blocks(g->device, o.grid_mult);
   const int pblocks = pull_grid(g->device, blocks, lane_policy(o.direction));
@@ -1944,6 +1946,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   P.tau = (long long)o.tau;
   P.max_levels = (long long)o.max_levels;
   P.nnz = (long long)g->nnz;
+  P.dense_grab = o.pull_grab > 0 ? o.pull_grab : kGrabDense;
   ms_init_seeds<<<1, 64, 0, st>>>(A, dsv, dsl, nseeds, P);
   MsCtrl hc;
   std::memset(&hc, 0, sizeof(hc));
@@ -2231,8 +2234,8 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   DeviceGraph *g = &h->g;
   cgbfs_options o;
   fill_options(opt, g->n, &o);
-  if (o.direction < 0 || o.direction > CGBFS_DIR_BEAMER_LANES)
-    return set_error(CGBFS_EINVAL, "unknown direction mode");
+  const int rc0 = check_options(&o);
+  if (rc0) return rc0;
   // lanes run at different depths: the union-count rule becomes the per-lane one
   if (o.direction == CGBFS_DIR_BEAMER) o.direction = CGBFS_DIR_BEAMER_LANES;
   CG_HIP(hipSetDevice(g->device));
@@ -2287,6 +2290,7 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   A.nroots = 64;
   A.act = ~0ull;
   A.symmetric = g->in_ptr == g->out_ptr ? 1 : 0;
+  A.seg_grab = o.seg_grab;
   MsParams &P = E->P;
   std::memset(&P, 0, sizeof(P));
   P.mode = o.direction;
@@ -2295,6 +2299,7 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   P.tau = (long long)o.tau;
   P.max_levels = 0x7FFFFFFFFFFFFFFFll;  // queries end on their own; levels are per query
   P.nnz = (long long)g->nnz;
+  P.dense_grab = o.pull_grab > 0 ? o.pull_grab : kGrabDense;
   E->tally = lane_policy(P.mode) ? 1 : 0;
   E->blocks = grid_blocks(g->device, o.grid_mult);
   E->pblocks = pull_grid(g->device, E->blocks, E->tally != 0);

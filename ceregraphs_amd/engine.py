@@ -65,7 +65,10 @@ class MSBFSResult:
 
 
 def _options(direction="beamer", alpha=None, beta=None, tau=None, grid_mult=None,
-             max_levels=None, rounds_hint=None, capture=False):
+             max_levels=None, rounds_hint=None, capture=False, pull_grab=None, seg_grab=None,
+             push_wave_degree=None, push_big_degree=None):
+  """cgbfs_options from keyword arguments; None or 0 takes the library's
+  default for that field (include/cgbfs.h)."""
   o = _lib.Options()
   if isinstance(direction, str):
     direction = _lib.DIRECTIONS[direction]
@@ -77,6 +80,10 @@ def _options(direction="beamer", alpha=None, beta=None, tau=None, grid_mult=None
   o.max_levels = int(max_levels or 0)
   o.rounds_hint = int(rounds_hint or 0)
   o.capture = 1 if capture else 0
+  o.pull_grab = int(pull_grab or 0)
+  o.seg_grab = int(seg_grab or 0)
+  o.push_wave_degree = int(push_wave_degree or 0)
+  o.push_big_degree = int(push_big_degree or 0)
   return o
 
 

@@ -63,20 +63,32 @@ extern "C" {
 
 typedef struct cgbfs_graph cgbfs_graph;
 
-/* Per-search options.  Zero-initialised fields take the defaults noted. */
+/* Per-search options.  Zero-initialised fields take the defaults noted.
+ * The last five are the kernels' tuning knobs (what the reference fixes at
+ * compile time in layout_bool.csl:18-38); an autotuner may set them per
+ * graph.  Values outside the ranges noted give CGBFS_EINVAL. */
 typedef struct {
   int32_t direction;  /* CGBFS_DIR_*                                  (0)    */
   int32_t rounds_hint; /* rounds enqueued before the first host poll (12) */
   double alpha;       /* push->pull when m_f > m_u / alpha            (14)   */
   double beta;        /* pull->push when n_f < n / beta               (24)   */
   int64_t tau;        /* CGBFS_DIR_TAU threshold on nf                (0.15n)*/
-  int32_t block;      /* threads per block for level kernels          (256)  */
-  int32_t grid_mult;  /* blocks per CU for grid-stride kernels (2 below 2^19
-                         vertices, 4 below 2^23, else 8)                    */
+  int32_t pull_grab;  /* 64-vertex chunks a wavefront takes per grab in a
+                         dense pull round, 1..1024 (16 single-root,
+                         1 multi-source)                                    */
+  int32_t grid_mult;  /* blocks per CU for grid-stride kernels, 1..64 (2 below
+                         2^19 vertices, 4 below 2^23, else 8)              */
   int64_t max_levels; /* abort with CGBFS_ETIMEOUT beyond this        (n+1)  */
   int32_t capture;    /* multi-source: capture the round kernels once into a
                          HIP graph and replay it, deciding every level on the
                          device (no per-round host poll, no per-round events) */
+  int32_t seg_grab;   /* multi-source: 256-entry hub-row segments a wavefront
+                         takes per grab, 1..1024                       (32)  */
+  int32_t push_wave_degree; /* single root: a wavefront expands each frontier
+                         vertex of out-degree >= this, 1..2^30          (32)  */
+  int32_t push_big_degree;  /* single root: the whole grid expands frontier
+                         vertices of out-degree >= this, >= push_wave_degree
+                         and <= 2^30                                    (256) */
 } cgbfs_options;
 
 /* Per-search statistics.  The caller owns the history arrays (capacity

@@ -107,3 +107,45 @@ def test_batch_searches_equal_oracle(directed, specs, per_graph, grid):
           np.testing.assert_array_equal(blev, lev, err_msg=f"graph {g} root {l} {cfg} {capture}")
           np.testing.assert_array_equal(bpar, par, err_msg=f"graph {g} root {l} {cfg} {capture}")
           assert m[g, l] == O.m_edges(gg, lev)
+
+
+class _FakeBatch:
+  """Stands in for an UnevenBatch: a pass costs a known function of the
+  configuration, so the tuner's search can be checked without a GPU."""
+
+  def __init__(self, cost, n=1 << 20):
+    self.cost, self.n, self.calls = cost, n, 0
+
+  def run(self, roots, device_out=None, **cfg):
+    self.calls += 1
+
+    class R:
+      pass
+    r = R()
+    r.total_ms = self.cost(cfg)
+    return r
+
+
+def test_autotune_coordinate_descent_finds_separable_optimum():
+  best_vals = {"direction": "beamer_lanes", "alpha": 48.0, "beta": 12.0, "grid_mult": 16,
+               "pull_grab": 2, "seg_grab": 128}
+
+  def cost(cfg):  # separable: each knob adds its own penalty
+    return 1.0 + sum(0.1 * (cfg[k] != v) + 0.01 * list(B.TUNE_SPACE[k]).index(cfg[k])
+                     for k, v in best_vals.items())
+  fb = _FakeBatch(cost)
+  best, table = B.autotune(fb, None, reps=1)
+  assert best == best_vals
+  assert table[0][0] == best_vals and table[0][1] == min(t for _, t in table)
+  # far fewer passes than the full product of the space
+  full = 1
+  for v in B.TUNE_SPACE.values():
+    full *= len(v)
+  assert fb.calls < full / 4
+
+
+def test_autotune_keeps_defaults_when_nothing_helps():
+  fb = _FakeBatch(lambda cfg: 5.0, n=1 << 20)
+  best, _ = B.autotune(fb, None, reps=1)
+  assert best["direction"] == "beamer" and best["alpha"] == 14.0 and best["beta"] == 24.0
+  assert best["grid_mult"] == 4 and best["pull_grab"] == 1 and best["seg_grab"] == 32

@@ -368,3 +368,38 @@ def test_graph_from_device_arrays_rejects_malformed(defect):
   with pytest.raises(_lib.CgbfsError) as err:
     DeviceGraph.from_csr(n, *t, symmetric=True)
   assert err.value.code == _lib.CGBFS_EINVAL
+
+
+def test_tuning_knobs_keep_results_and_are_checked():
+  """The work-distribution knobs (chunks per grab, hub segments per grab,
+  the push tier thresholds) only move work between wavefronts: any setting
+  gives bit-exact results.  Out-of-range values get CGBFS_EINVAL."""
+  from ceregraphs_amd import _lib
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(14, 16, 3)
+  g = O.Graph(n, ip, ii, op, oi, True)
+  deg = np.diff(op)
+  with DeviceGraph.from_rmat(14, 16, 3) as dg:
+    hub = int(np.argmax(deg))
+    for kw in [dict(pull_grab=1), dict(pull_grab=64), dict(push_wave_degree=1, push_big_degree=2),
+               dict(push_wave_degree=8, push_big_degree=64),
+               dict(push_wave_degree=1 << 30, push_big_degree=1 << 30)]:
+      for s in (hub, 7):
+        for d in ("beamer", "topdown", "bottomup"):
+          _check_single(dg, g, s, d, **kw)
+    roots = np.random.default_rng(5).choice(np.nonzero(deg)[0], 64, replace=False)
+    roots[0] = hub
+    expect = [O.bfs(g, int(s)) for s in roots]
+    for kw in [dict(pull_grab=3, seg_grab=1), dict(pull_grab=16, seg_grab=1024)]:
+      for cap in (False, True):
+        r = dg.msbfs(roots, capture=cap, **kw)
+        for l in range(64):
+          np.testing.assert_array_equal(r.levels[l], expect[l][0].astype(np.int16), err_msg=f"{kw} lane {l}")
+          np.testing.assert_array_equal(r.parents[l], expect[l][1], err_msg=f"{kw} lane {l}")
+    for bad in [dict(pull_grab=2000), dict(seg_grab=5000), dict(grid_mult=100),
+                dict(push_wave_degree=64, push_big_degree=32)]:
+      with pytest.raises(_lib.CgbfsError) as e:
+        dg.bfs(hub, **bad)
+      assert e.value.code == _lib.CGBFS_EINVAL, bad
+      with pytest.raises(_lib.CgbfsError) as e:
+        dg.msbfs(roots[:4], **bad)
+      assert e.value.code == _lib.CGBFS_EINVAL, bad
