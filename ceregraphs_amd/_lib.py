@@ -22,6 +22,10 @@ DIR_TOPDOWN = 1
 DIR_BOTTOMUP = 2
 DIR_TAU = 3
 DIR_BEAMER_LANES = 4
+LOOP_AUTO = 0
+LOOP_KERNELS = 1
+LOOP_PERSISTENT = 2
+LOOPS = {"auto": LOOP_AUTO, "kernels": LOOP_KERNELS, "persistent": LOOP_PERSISTENT}
 DIRECTIONS = {"beamer": DIR_BEAMER, "topdown": DIR_TOPDOWN, "bottomup": DIR_BOTTOMUP,
               "tau": DIR_TAU, "beamer_lanes": DIR_BEAMER_LANES}
 
@@ -57,6 +61,7 @@ class Options(ctypes.Structure):
       ("seg_grab", ctypes.c_int32),
       ("push_wave_degree", ctypes.c_int32),
       ("push_big_degree", ctypes.c_int32),
+      ("loop", ctypes.c_int32),
   ]
 
 

@@ -20,6 +20,8 @@
 // Q[d&1]/F[d%3], writes Q[(d+1)&1]/F[(d+1)%3] and zeroes F[(d+2)%3] (the
 // level d-1 bitmap, no longer read), so every bitmap is clear one round
 // before it is written.
+#include <hip/hip_cooperative_groups.h>
+
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -546,9 +548,11 @@ __device__ __forceinline__ void pull_prefetch(const LevelArgs &A, int32_t ch, bo
   }
 }
 
-__global__ void __launch_bounds__(kBlock, 8) bfs_pull_round(LevelArgs A, int d) {
+// One pull round over the whole graph by the calling grid (every thread of
+// every block must call it): the body of bfs_pull_round, and of a pull level
+// of the persistent search.
+__device__ void pull_level(const LevelArgs &A, int d) {
   Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 1) return;
   __shared__ unsigned long long red[kWavesPerBlock];
   const uint32_t *fcur = A.fb[d % 3];
   uint32_t *fnext = A.fb[(d + 1) % 3];
@@ -731,17 +735,21 @@ __global__ void __launch_bounds__(kBlock, 8) bfs_pull_round(LevelArgs A, int d) 
   if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
+__global__ void __launch_bounds__(kBlock, 8) bfs_pull_round(LevelArgs A, int d) {
+  if (ld_ctrl(&A.ctrl->done) || ld_ctrl(&A.ctrl->dir) != 1) return;
+  pull_level(A, d);
+}
+
 // ---------------------------------------------------------------------------
 // Decide: termination, direction, history (one thread)
 // ---------------------------------------------------------------------------
-__global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The end of round d (one thread): records the round, resets the round
+// counters and chooses the next round's direction, or ends the search.
+// nf is the number of vertices the round reached.
+__device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsParams &P,
+                             long long nf) {
   Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done)) return;
-  const int nxt = (d + 1) & 1;
   const int cur = ld_ctrl(&c->dir);
-  // a push round listed its new vertices, a pull round counted them
-  const long long nf = cur == 0 ? (long long)ld_ctrl(&c->qtail[nxt]) : (long long)ld_ctrl(&c->nf);
   const long long mf = (long long)ld_ctrl(&c->mf);
   const long long mu = ld_ctrl(&c->mu) - (long long)ld_ctrl(&c->in_sum);
   unsigned long long now = wall_clock64();
@@ -792,6 +800,161 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
   if (d + 1 >= P.max_levels) {
     st_ctrl(&c->status, (int)CGBFS_ETIMEOUT);
     st_ctrl(&c->done, 1);
+  }
+}
+
+__global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctrl *c = A.ctrl;
+  if (ld_ctrl(&c->done)) return;
+  // a push round listed its new vertices, a pull round counted them
+  const long long nf = ld_ctrl(&c->dir) == 0 ? (long long)ld_ctrl(&c->qtail[(d + 1) & 1])
+                                             : (long long)ld_ctrl(&c->nf);
+  decide_round(A, d, n, P, nf);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent search: every level in one cooperative launch
+// ---------------------------------------------------------------------------
+
+// Offers frontier vertex u as predecessor of v in round d, as push_visit,
+// for a push that keeps no queue: the new frontier is the bitmap alone.
+__device__ __forceinline__ void push_visit_bm(const LevelArgs &A, int d, uint32_t u, int32_t v,
+                                              uint32_t *fnext, unsigned long long &mf,
+                                              unsigned long long &in_sum,
+                                              unsigned long long &found) {
+  const uint32_t w = (uint32_t)v >> 5, b = 1u << (v & 31);
+  const uint32_t vw = __builtin_nontemporal_load(&A.visited[w]);
+  if (!(vw & b)) {
+    const uint32_t old = atomicOr(&A.visited[w], b);
+    if (!(old & b)) {
+      A.levels[v] = d + 1;
+      atomicOr(&fnext[w], b);
+      found++;
+      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
+      in_sum += (unsigned long long)(A.in_ptr[v + 1] - A.in_ptr[v]);
+    }
+    atomicMin(&A.parents[v], u);
+  } else {
+    const int lv = __hip_atomic_load(&A.levels[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lv == d + 1 || lv == -1) atomicMin(&A.parents[v], u);
+  }
+}
+
+// One push round read from the frontier bitmap by the calling grid (every
+// thread of every block must call it): wavefronts take 64-vertex chunks of
+// the bitmap; a frontier vertex of out-degree below wave_degree is expanded
+// by its own lane, a larger one by the whole wavefront, 64 out-neighbours a
+// step.  Meant for graphs small enough that no whole-grid tier is needed.
+__device__ void push_level_bitmap(const LevelArgs &A, int d) {
+  Ctrl *c = A.ctrl;
+  __shared__ unsigned long long red[kWavesPerBlock];
+  const uint32_t *fcur = A.fb[d % 3];
+  uint32_t *fnext = A.fb[(d + 1) % 3];
+  uint32_t *fclear = A.fb[(d + 2) % 3];
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t w = gtid; w < A.nwords; w += gstride) fclear[w] = 0;
+  const int lane = lane_id();
+  const int64_t wave = gtid / kWave, nwaves = gstride / kWave;
+  const int64_t nchunks = A.nwords / 2;
+  unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0, found = 0;
+  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
+    const uint32_t word = fcur[ch * 2 + (lane >> 5)];
+    const int64_t v = ch * kWave + lane;
+    const bool on = v < A.n && ((word >> (lane & 31)) & 1u);
+    if (!__any(on)) continue;
+    const int64_t beg = on ? A.out_ptr[v] : 0, end = on ? A.out_ptr[v + 1] : 0;
+    const bool small = on && end - beg < A.wave_degree;
+    const bool big = on && end - beg >= A.big_degree;
+    if (on && !big) {
+      rows++;
+      scanned += (unsigned long long)(end - beg);
+    }
+    // the whole grid expands the big ones after the next barrier
+    const int64_t slot = wave_append(&c->big_tail, big);
+    if (big) A.bigq[slot] = (int32_t)v;
+    if (small)
+      for (int64_t e = beg; e < end; e++)
+        push_visit_bm(A, d, (uint32_t)v, A.out_idx[e], fnext, mf, in_sum, found);
+    uint64_t wide = ballot64(on && !small && !big);
+    while (wide) {
+      const int j = __ffsll((unsigned long long)wide) - 1;
+      wide &= wide - 1;
+      const int64_t bj = __shfl(beg, j), ej = __shfl(end, j);
+      const uint32_t uj = (uint32_t)(ch * kWave + j);
+      for (int64_t e = bj + lane; e < ej; e += kWave)
+        push_visit_bm(A, d, uj, A.out_idx[e], fnext, mf, in_sum, found);
+    }
+  }
+  unsigned long long s;
+  s = block_sum(found, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->nf, s);
+  s = block_sum(mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  s = block_sum(in_sum, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
+  s = block_sum(scanned, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  s = block_sum(rows, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
+}
+
+// The big frontier vertices a bitmap push listed (out-degree >= big_degree),
+// expanded by the whole grid one after another, every thread taking a
+// strided share of each row.  Every thread of every block must call it.
+__device__ void push_big_rows(const LevelArgs &A, int d) {
+  Ctrl *c = A.ctrl;
+  __shared__ unsigned long long red[kWavesPerBlock];
+  uint32_t *fnext = A.fb[(d + 1) % 3];
+  const int64_t nb = (int64_t)ld_ctrl(&c->big_tail);
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long mf = 0, in_sum = 0, scanned = 0, found = 0;
+  for (int64_t j = 0; j < nb; j++) {
+    const int32_t u = A.bigq[j];
+    const int64_t beg = A.out_ptr[u], end = A.out_ptr[u + 1];
+    for (int64_t e = beg + gtid; e < end; e += gstride)
+      push_visit_bm(A, d, (uint32_t)u, A.out_idx[e], fnext, mf, in_sum, found);
+    if (gtid == 0) scanned += (unsigned long long)(end - beg);
+  }
+  unsigned long long s;
+  s = block_sum(found, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->nf, s);
+  s = block_sum(mf, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+  s = block_sum(in_sum, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
+  s = block_sum(scanned, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  if (gtid == 0 && nb) atomicAdd(&c->rows, (unsigned long long)nb);
+}
+
+// The whole level loop in one launch (cooperative: every block is resident,
+// so grid barriers are safe).  Each level is one round by the whole grid,
+// a grid barrier, the decide by one thread, and a second barrier, after
+// which every block reads the same done flag and direction: all blocks run
+// the same number of levels and reach every barrier.  The loop is bounded
+// by max_levels, and the decide ends the search there at the latest.  Pull
+// rounds are pull_level; push rounds read the frontier bitmap, so no queue
+// is kept, and expand their big vertices in a second phase behind one more
+// barrier (taken by every block: the direction is uniform).  For small graphs, where the per-round launches of the level
+// kernels cost more than the rounds themselves.
+__global__ void __launch_bounds__(kBlock, 8) bfs_persistent(LevelArgs A, int64_t n, BfsParams P) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  Ctrl *c = A.ctrl;
+  for (long long d = 0; d <= P.max_levels; d++) {
+    if (ld_ctrl(&c->done)) break;
+    if (ld_ctrl(&c->dir) == 0) {  // uniform: every block read the same direction
+      push_level_bitmap(A, (int)d);
+      grid.sync();
+      push_big_rows(A, (int)d);
+    } else {
+      pull_level(A, (int)d);
+    }
+    grid.sync();
+    if (blockIdx.x == 0 && threadIdx.x == 0) decide_round(A, (int)d, n, P, (long long)ld_ctrl(&c->nf));
+    grid.sync();
   }
 }
 
@@ -861,12 +1024,39 @@ int check_options(const cgbfs_options *o) {
   if (o->push_big_degree < o->push_wave_degree || o->push_big_degree > (1 << 30))
     return set_error(CGBFS_EINVAL, "push_big_degree must be in [push_wave_degree, 2^30]");
   if (!(o->alpha > 0) || !(o->beta > 0)) return set_error(CGBFS_EINVAL, "alpha and beta must be positive");
+  if (o->loop < CGBFS_LOOP_AUTO || o->loop > CGBFS_LOOP_PERSISTENT)
+    return set_error(CGBFS_EINVAL, "unknown loop mode");
   return CGBFS_OK;
 }
 
 }  // namespace cg
 
 using namespace cg;
+
+// Graphs up to this many vertices run their levels in one persistent launch
+// by default (CGBFS_LOOP_AUTO).  Measured on R-MAT ef16 searches, device time
+// per search, per-level kernels / persistent: s10 0.19 / 0.10 ms, s12 0.22 /
+// 0.13, s14 0.26 / 0.22, s16 0.31 / 0.48, s18 0.37 / 1.03, s21 0.60 / 2.32:
+// beyond 2^14 vertices the grid barriers (two or three per level, over
+// hundreds of blocks) cost more than the launches they replace.
+constexpr int64_t kPersistentMaxN = (int64_t)1 << 14;
+
+// Caps *blocks at what the device holds resident at once for bfs_persistent.
+static int persistent_grid(int device, int *blocks) {
+  static std::mutex mu;
+  static int occ_of[kMaxDevices];
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0 || device >= kMaxDevices) return set_error(CGBFS_EINVAL, "device index out of range");
+  if (!occ_of[device]) {
+    int occ = 0, cus = 0;
+    CG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, bfs_persistent, kBlock, 0));
+    CG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (occ <= 0 || cus <= 0) return set_error(CGBFS_EHIP, "no resident blocks for the persistent search");
+    occ_of[device] = occ * cus;
+  }
+  *blocks = std::max(1, std::min(*blocks, occ_of[device]));
+  return CGBFS_OK;
+}
 
 // The search itself, shared by cgbfs_bfs and cgbfs_bfs_noalloc: outputs go
 // to device arrays par / lev (the caller's or the workspace's).  events: time
@@ -912,38 +1102,54 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
   bfs_init_source<<<1, 64, 0, st>>>(A, source, P, (long long)g->total_out_edges);
   CG_HIP(hipGetLastError());
 
-  // Rounds are enqueued in chunks without host synchronisation; kernels of
-  // rounds past the end see ctrl->done and return at once.  The host polls
-  // the done flag once per chunk.  The search time is the device clock from
-  // bfs_start to the decide of the last round (ctrl->t0 .. ctrl->t_end), so
-  // neither the gated launches after the end nor the host poll count.
-  int64_t d = 0;
-  int chunk = o.rounds_hint > 0 ? o.rounds_hint : 12;
   Ctrl hc;
   hipEvent_t rev[2 * 64];  // per-round events of the first 64 rounds
   int nrev = 0;
-  for (;;) {
-    for (int k = 0; k < chunk; k++, d++) {
-      hipEvent_t ra = nullptr, rb = nullptr;
-      if (events && nrev + 2 <= 2 * 64 && (ra = pool.make()) && (rb = pool.make())) {
-        rev[nrev++] = ra;
-        rev[nrev++] = rb;
-        hipEventRecord(ra, st);
-      }
-      bfs_compact<<<blocks, kBlock, 0, st>>>(A, (int)d);
-      bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
-      bfs_push_scan<<<1, 1024, 0, st>>>(A);
-      bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);
-      bfs_pull_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
-      if (rb) hipEventRecord(rb, st);
-      bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n, P);
-    }
-    CG_HIP(hipGetLastError());
+  const bool persistent = o.loop == CGBFS_LOOP_PERSISTENT ||
+                          (o.loop == CGBFS_LOOP_AUTO && g->n <= kPersistentMaxN);
+  if (persistent) {
+    // One cooperative launch: as many blocks as give every wavefront a
+    // chunk, at most the usual grid and at most what is resident at once.
+    const int64_t nch = A.nwords / 2;
+    int pblocks = (int)std::min<int64_t>(blocks, (nch + kWavesPerBlock - 1) / kWavesPerBlock);
+    int rc = persistent_grid(g->device, &pblocks);
+    if (rc) return rc;
+    int64_t n_arg = g->n;
+    void *args[] = {(void *)&A, (void *)&n_arg, (void *)&P};
+    CG_HIP(hipLaunchCooperativeKernel((const void *)bfs_persistent, dim3(pblocks), dim3(kBlock), args, 0, st));
     CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     CG_HIP(hipStreamSynchronize(st));
-    if (hc.done) break;
-    if (d > o.max_levels + chunk) break;
-    chunk = 4;
+  } else {
+    // Rounds are enqueued in chunks without host synchronisation; kernels of
+    // rounds past the end see ctrl->done and return at once.  The host polls
+    // the done flag once per chunk.  The search time is the device clock from
+    // bfs_start to the decide of the last round (ctrl->t0 .. ctrl->t_end), so
+    // neither the gated launches after the end nor the host poll count.
+    int64_t d = 0;
+    int chunk = o.rounds_hint > 0 ? o.rounds_hint : 12;
+    for (;;) {
+      for (int k = 0; k < chunk; k++, d++) {
+        hipEvent_t ra = nullptr, rb = nullptr;
+        if (events && nrev + 2 <= 2 * 64 && (ra = pool.make()) && (rb = pool.make())) {
+          rev[nrev++] = ra;
+          rev[nrev++] = rb;
+          hipEventRecord(ra, st);
+        }
+        bfs_compact<<<blocks, kBlock, 0, st>>>(A, (int)d);
+        bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
+        bfs_push_scan<<<1, 1024, 0, st>>>(A);
+        bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);
+        bfs_pull_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
+        if (rb) hipEventRecord(rb, st);
+        bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n, P);
+      }
+      CG_HIP(hipGetLastError());
+      CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipStreamSynchronize(st));
+      if (hc.done) break;
+      if (d > o.max_levels + chunk) break;
+      chunk = 4;
+    }
   }
   if (hc.status) return set_error(hc.status, "BFS exceeded max_levels");
   const double hz = wall_clock_hz(g->device);
@@ -1052,5 +1258,6 @@ extern "C" int cgbfs_bfs_noalloc(cgbfs_graph *h, int64_t source, int32_t *parent
   if (rc) return rc;
   Workspace ws;
   ws1_layout(g->n, false, (char *)base, &ws);
+  o.loop = CGBFS_LOOP_KERNELS;  // the cooperative launch is the runtime's; keep to plain launches
   return bfs_run(g, ws, source, levels, (uint32_t *)parents, o, stats, false);
 }

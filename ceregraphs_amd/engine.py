@@ -66,7 +66,7 @@ class MSBFSResult:
 
 def _options(direction="beamer", alpha=None, beta=None, tau=None, grid_mult=None,
              max_levels=None, rounds_hint=None, capture=False, pull_grab=None, seg_grab=None,
-             push_wave_degree=None, push_big_degree=None):
+             push_wave_degree=None, push_big_degree=None, loop=None):
   """cgbfs_options from keyword arguments; None or 0 takes the library's
   default for that field (include/cgbfs.h)."""
   o = _lib.Options()
@@ -84,6 +84,7 @@ def _options(direction="beamer", alpha=None, beta=None, tau=None, grid_mult=None
   o.seg_grab = int(seg_grab or 0)
   o.push_wave_degree = int(push_wave_degree or 0)
   o.push_big_degree = int(push_big_degree or 0)
+  o.loop = _lib.LOOPS[loop] if isinstance(loop, str) else int(loop or 0)
   return o
 
 

@@ -89,7 +89,15 @@ typedef struct {
   int32_t push_big_degree;  /* single root: the whole grid expands frontier
                          vertices of out-degree >= this, >= push_wave_degree
                          and <= 2^30                                    (256) */
+  int32_t loop;       /* single root: CGBFS_LOOP_* -- how the levels run  (0) */
 } cgbfs_options;
+
+/* How a single-root search runs its levels (cgbfs_options.loop). */
+#define CGBFS_LOOP_AUTO 0       /* persistent up to 2^14 vertices, else kernels */
+#define CGBFS_LOOP_KERNELS 1    /* a fixed launch sequence per level, each kernel
+                                   gated on the device's decision               */
+#define CGBFS_LOOP_PERSISTENT 2 /* one cooperative launch runs every level, grid
+                                   barriers between a level and its decision    */
 
 /* Per-search statistics.  The caller owns the history arrays (capacity
  * max_hist entries each; any of them may be NULL).  Per-round meaning

@@ -403,3 +403,31 @@ def test_tuning_knobs_keep_results_and_are_checked():
       with pytest.raises(_lib.CgbfsError) as e:
         dg.msbfs(roots[:4], **bad)
       assert e.value.code == _lib.CGBFS_EINVAL, bad
+
+
+@pytest.mark.parametrize("loop", ["persistent", "kernels"])
+def test_level_loops_equal_oracle(loop):
+  """Both ways of running the levels -- one cooperative launch with grid
+  barriers, or the fixed per-level launch sequence -- give the oracle's
+  levels, lowest-id parents, per-round counts and m, on undirected and
+  directed graphs, under every direction policy, on a graph with a
+  1000-level path (1000 rounds of barriers) and with max_levels hit."""
+  from ceregraphs_amd import _lib
+  for scale, ef, seed, directed in [(10, 16, 0, False), (14, 8, 2, False), (13, 8, 10, True),
+                                    (17, 16, 4, False)]:
+    abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+    n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+    g = O.Graph(n, ip, ii, op, oi, not directed)
+    deg = np.diff(op)
+    with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
+      for s in (int(np.argmax(deg)), int(np.nonzero(deg)[0][-1]), 0):
+        for d in DIRECTIONS:
+          _check_single(dg, g, s, d, loop=loop)
+  edges = [(i, i + 1) for i in range(1000)] + [(500, j) for j in range(1001, 1400)]
+  g = _edges_graph(edges, 1500, directed=False)
+  with _device(g) as dg:
+    for d in DIRECTIONS:
+      _check_single(dg, g, 0, d, loop=loop)
+    with pytest.raises(_lib.CgbfsError) as e:
+      dg.bfs(0, loop=loop, max_levels=10)
+    assert e.value.code == _lib.CGBFS_ETIMEOUT
