@@ -1999,7 +1999,10 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     }
     if (!md->exec) {
       hipGraph_t graph = nullptr;
-      e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+      // Relaxed: searches on other graph handles may run from other host
+      // threads meanwhile, and their synchronous calls must not invalidate
+      // this capture (it only enqueues this handle's kernels).
+      e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
       if (e == hipSuccess) {
         for (int r = 0; r < kGraphRounds; r++) {
           pull_part(true);
@@ -2312,7 +2315,7 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   // itself on the control block): capture it once, with the copies of the
   // control block and the m counters back to pinned memory.
   hipGraph_t graph = nullptr;
-  e = hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal);
+  e = hipStreamBeginCapture(E->st, hipStreamCaptureModeRelaxed);
   if (e == hipSuccess) {
     eng_round(E);
     e = hipMemcpyAsync(&E->pin->hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, E->st);

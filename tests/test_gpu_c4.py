@@ -22,7 +22,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 @pytest.fixture(scope="module")
 def c4():
   batch = B.UnevenBatch.from_rmat(B.UNEVEN_SPECS, balance_grid=B.UNEVEN_BALANCE_GRID,
-                                  keep_host=True)
+                                  keep_host=True, keep_parts=True)
   roots = B.pick_roots(batch.graph_out_degrees(), B.UNEVEN_ROOTS_PER_GRAPH, B.UNEVEN_ROOT_SEED)
   res = batch.run(roots, capture=True)
   levels, parents = np.asarray(res.levels), np.asarray(res.parents)
@@ -62,3 +62,26 @@ def test_c4_autotuned_pass_equals_default(c4):
   tlev, tpar, cfg = c4["tuned"]
   np.testing.assert_array_equal(tlev, c4["levels"], err_msg=str(cfg))
   np.testing.assert_array_equal(tpar, c4["parents"], err_msg=str(cfg))
+
+
+def test_c4_concurrent_per_graph_passes_equal_union(c4):
+  """Every graph's own pass, all eight in flight at once on their own
+  streams, each graph with its own direction policy and alpha / beta, gives
+  exactly the union pass's levels and parents."""
+  batch, roots = c4["batch"], c4["roots"]
+  cfgs = [{"direction": ("beamer", "beamer_lanes", "bottomup", "topdown")[g % 4],
+           "alpha": (4.0, 14.0, 48.0)[g % 3], "beta": (12.0, 24.0, 96.0)[(g + 1) % 3]}
+          for g in range(len(batch.graphs))]
+  conc = B.ConcurrentPasses(batch.graphs, 32)
+  try:
+    for cap in (True, False):
+      conc.run(roots, cfgs, capture=cap)
+      for g in range(len(batch.graphs)):
+        lo, hi = int(batch.offsets[g]), int(batch.offsets[g + 1])
+        lev = conc.outs[g][1].view(32, hi - lo).cpu().numpy()
+        par = conc.outs[g][0].view(32, hi - lo).cpu().numpy().astype(np.int64)
+        np.testing.assert_array_equal(lev, c4["levels"][:, lo:hi], err_msg=f"graph {g} {cfgs[g]}")
+        upar = c4["parents"][:, lo:hi].astype(np.int64)
+        np.testing.assert_array_equal(par, np.where(upar >= 0, upar - lo, -1), err_msg=f"graph {g}")
+  finally:
+    conc.close()
