@@ -1803,6 +1803,13 @@ __global__ void eng_seed(MsArgs A, LaneOut *out, SeedSet S, MsParams P) {
   const uint64_t added = wave_or64(mine);
   const uint64_t pull = wave_or64(dir ? mine : 0ull);
   if (t == 0 && added) {
+    if (ld_ctrl(&c->done)) {
+      // An idle engine starts the union-count policy afresh, as a one-call
+      // search does: pushing (unless every round pulls), with the seeds as
+      // the previous frontier.
+      st_ctrl(&c->gdir, P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0);
+      st_ctrl(&c->prev_nf, (long long)S.count);
+    }
     st_ctrl(&c->pmask, (unsigned long long)(ld_ctrl(&c->pmask) | pull));
     st_ctrl(&c->qmask, (unsigned long long)(ld_ctrl(&c->qmask) | (added & ~pull)));
     st_ctrl(&c->listed, 0);

@@ -10,12 +10,20 @@ query at once.  Engines of different graphs run on their own streams and
 overlap on the GPU.  This is continuous batching: queries are admitted
 while others are running, not collected into closed batches.
 
+Admission "waves" instead lets an engine take queries only while it has none
+in flight: everything that arrived meanwhile (up to 64) starts in one round,
+so the lanes stay at one depth and the engine runs the union-count direction
+policy (every lane pushes or pulls together).  A round then costs what a
+round of a closed 64-root pass costs, about 2.5x less than a round over
+lanes at different depths, at the price of waiting for the wave in flight.
+
 Result buffers (parents int32[n] + levels int16[n]) come from per-graph
 pools sized within the HBM budget together with the graphs and the engine
 workspaces.  A buffer goes back to its pool when the query completes, so
 this loop measures serving the search, not holding results.
 
     python -m ceregraphs_amd.serve --rates 1000,10000,50000 --queries 3000
+    python -m ceregraphs_amd.serve --admission waves
 
 The plain loop for comparison runs the same stream one query per launch
 (cgbfs_bfs) in arrival order.  Latency = completion - arrival (wall clock);
@@ -37,6 +45,8 @@ SERVED_SPECS = [(16, 16, 21), (18, 16, 22), (20, 16, 23), (22, 16, 24), (24, 16,
                 (26, 16, 26)]
 STREAM_SEED = 20
 HBM_BUDGET = 200 << 30
+# admission policy -> the engines' direction policy
+ADMISSION_DIRECTION = {"continuous": "beamer_lanes", "waves": "beamer"}
 
 
 @dataclass
@@ -75,12 +85,24 @@ class GraphSlot:
     self.dg = dg
     self.n = dg.n
     self.engine = ServingEngine(dg, **engine_opts)
+    self.engine_opts = dict(engine_opts)
     self.free_bufs = [(torch.empty(self.n, dtype=torch.int32, device="cuda"),
                        torch.empty(self.n, dtype=torch.int16, device="cuda")) for _ in range(buffers)]
     self.free_lanes = list(range(ServingEngine.LANES))
     self.lane_query = {}
     self.inflight = False
     self.waiting = []  # arrived query ids, FIFO
+    self.admitting = False  # a wave is being admitted (admission "waves")
+
+  def set_engine(self, **engine_opts):
+    """Replaces the engine (none may be in flight), e.g. for another direction
+    policy; the result pool stays."""
+    if self.lane_query or self.inflight:
+      raise RuntimeError("the engine has queries in flight")
+    if engine_opts != self.engine_opts:
+      self.engine.close()
+      self.engine = ServingEngine(self.dg, **engine_opts)
+      self.engine_opts = dict(engine_opts)
 
   def close(self):
     self.engine.close()
@@ -112,11 +134,15 @@ def build_slots(specs, budget, buffers_cap=ServingEngine.LANES, **engine_opts):
                  "result_bytes": int(nbuf * per_buf)}
 
 
-def serve_stream(slots, stream, deadline_s=120.0, on_complete=None):
+def serve_stream(slots, stream, deadline_s=120.0, on_complete=None, admission="continuous"):
   """Runs the stream through the engines; returns per-query latency (s),
   m per query, and the wall span from the first arrival to the last
   completion.  on_complete(q, (parents, levels), m, rounds), if given, sees
-  each query's result buffers before they go back to the pool."""
+  each query's result buffers before they go back to the pool.  admission:
+  "continuous" (a query takes a free lane at the next round) or "waves" (an
+  engine takes queries only while none is in flight)."""
+  if admission not in ADMISSION_DIRECTION:
+    raise ValueError(f"admission must be one of {sorted(ADMISSION_DIRECTION)}")
   nq = len(stream.t)
   lat = np.full(nq, np.nan)
   m = np.zeros(nq, np.int64)
@@ -148,13 +174,18 @@ def serve_stream(slots, stream, deadline_s=120.0, on_complete=None):
           s.free_lanes.append(lane)
           s.free_bufs.append(buf)
           done += 1
-      # admit arrived queries into free lanes, then run a round if any is busy
-      while s.waiting and s.free_lanes and s.free_bufs:
+      # admit arrived queries into free lanes (waves: only into an engine
+      # with none in flight), then run a round if any lane is busy
+      while (s.waiting and s.free_lanes and s.free_bufs and
+             (admission == "continuous" or len(s.free_lanes) == ServingEngine.LANES or
+              s.admitting)):
+        s.admitting = True
         q = s.waiting.pop(0)
         lane = s.free_lanes.pop(0)
         buf = s.free_bufs.pop()
         s.engine.submit(lane, int(stream.root[q]), buf[0], buf[1])
         s.lane_query[lane] = (q, buf)
+      s.admitting = False
       if s.lane_query:
         s.engine.step()
         s.inflight = True
@@ -195,7 +226,14 @@ def main(argv=None):
   p.add_argument("--max-scale", type=int, default=26, help="serve only graphs up to this scale")
   p.add_argument("--budget-gib", type=float, default=HBM_BUDGET / 2**30)
   p.add_argument("--no-plain", action="store_true")
+  p.add_argument("--admission", default="continuous,waves",
+                 help="continuous, waves, or both (comma-separated; each with its "
+                      "direction policy: continuous beamer_lanes, waves beamer)")
   args = p.parse_args(argv)
+  admissions = args.admission.split(",")
+  for a in admissions:
+    if a not in ADMISSION_DIRECTION:
+      p.error(f"unknown admission {a}")
   import torch
 
   specs = [s for s in SERVED_SPECS if s[0] <= args.max_scale]
@@ -208,9 +246,14 @@ def main(argv=None):
             "rates": {}}
   for rate in [float(x) for x in args.rates.split(",")]:
     stream = poisson_stream(rate, args.queries, nonisolated)
-    lat, m, span = serve_stream(slots, stream)
-    line = {"engine": dict(latency_summary(lat), gteps=float(m.sum() / span / 1e9),
-                           qps=float(len(lat) / span))}
+    line = {}
+    for a in admissions:
+      for s in slots:
+        s.set_engine(direction=ADMISSION_DIRECTION[a])
+      lat, m, span = serve_stream(slots, stream, admission=a)
+      key = "engine" if a == "continuous" else "engine_" + a
+      line[key] = dict(latency_summary(lat), gteps=float(m.sum() / span / 1e9),
+                       qps=float(len(lat) / span))
     if not args.no_plain:
       plat, pm, pspan = plain_loop(slots, stream)
       if not np.array_equal(pm, m):

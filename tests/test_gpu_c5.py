@@ -4,12 +4,15 @@ serving engine each (continuous batching), result pools sized within the
 200 GB HBM budget, and a Poisson stream of queries at 1000 queries/s (seed
 20), each picking a graph and then a non-isolated root uniformly.
 
-Every query's parents and levels are kept as it completes.  At s16-s22 each
-query is checked against the oracle (levels, lowest-id parents, m, rounds);
-at s24 and s26 each query equals the same query run alone (cgbfs_bfs), and
-two queries per graph are also checked against the oracle.  The graphs are
-built and the stream served once (module fixture); each graph is then its
-own test."""
+The stream is served twice: with continuous batching (per-lane direction
+policies) and with admission in waves (an engine takes queries only while
+none is in flight; union-count policy).  Every query's parents and levels
+are kept as it completes.  At s16-s22 each query is checked against the
+oracle (levels, lowest-id parents, m, rounds); at s24 and s26 each query
+equals the same query run alone (cgbfs_bfs), and two queries per graph are
+also checked against the oracle.  The graphs are built and the stream
+served once per admission (module fixture); each graph and admission is
+then its own test."""
 import numpy as np
 import pytest
 
@@ -19,6 +22,7 @@ from ceregraphs_amd import serve
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 
 QUERIES = 240
+ADMISSIONS = ["continuous", "waves"]
 
 
 @pytest.fixture(scope="module")
@@ -27,26 +31,33 @@ def c5():
   slots, mem = serve.build_slots(serve.SERVED_SPECS, serve.HBM_BUDGET)
   nonisolated = [np.nonzero(s.dg.out_degree() > 0)[0] for s in slots]
   stream = serve.poisson_stream(1000.0, QUERIES, nonisolated, seed=serve.STREAM_SEED)
-  kept = {}
+  runs = {}
+  for admission in ADMISSIONS:
+    for s in slots:
+      s.set_engine(direction=serve.ADMISSION_DIRECTION[admission])
+    kept = {}
 
-  def keep(q, buf, m, rounds):
-    kept[q] = (buf[0].clone(), buf[1].clone(), m, rounds)
+    def keep(q, buf, m, rounds, kept=kept):
+      kept[q] = (buf[0].clone(), buf[1].clone(), m, rounds)
 
-  lat, m, span = serve.serve_stream(slots, stream, deadline_s=300.0, on_complete=keep)
-  torch.cuda.synchronize()
-  yield {"slots": slots, "stream": stream, "kept": kept, "lat": lat, "m": m, "mem": mem}
+    lat, m, span = serve.serve_stream(slots, stream, deadline_s=300.0, on_complete=keep,
+                                      admission=admission)
+    torch.cuda.synchronize()
+    runs[admission] = {"kept": kept, "lat": lat, "m": m}
+  yield {"slots": slots, "stream": stream, "runs": runs, "mem": mem}
   for s in slots:
     s.close()
     s.dg.close()
 
 
-def test_c5_stream_completed(c5):
-  st = c5["stream"]
-  assert len(c5["kept"]) == QUERIES and np.isfinite(c5["lat"]).all()
+@pytest.mark.parametrize("admission", ADMISSIONS)
+def test_c5_stream_completed(c5, admission):
+  st, run = c5["stream"], c5["runs"][admission]
+  assert len(run["kept"]) == QUERIES and np.isfinite(run["lat"]).all()
   assert c5["mem"]["result_buffers_per_graph"] >= 1
   assert c5["mem"]["graphs_and_engines_bytes"] + c5["mem"]["result_bytes"] <= serve.HBM_BUDGET
   for q in range(QUERIES):
-    assert c5["kept"][q][2] == c5["m"][q]
+    assert run["kept"][q][2] == run["m"][q]
   # every graph was queried
   assert set(st.graph.tolist()) == set(range(len(serve.SERVED_SPECS)))
 
@@ -56,11 +67,12 @@ def _oracle_graph(dg):
   return O.Graph(dg.n, ip, ii, op, oi, True)
 
 
+@pytest.mark.parametrize("admission", ADMISSIONS)
 @pytest.mark.parametrize("gi", range(len(serve.SERVED_SPECS)))
-def test_c5_queries(c5, gi):
+def test_c5_queries(c5, gi, admission):
   """s16-s22: every query == the oracle.  s24, s26: every query == the
   same query alone, two of them == the oracle."""
-  st, kept = c5["stream"], c5["kept"]
+  st, kept = c5["stream"], c5["runs"][admission]["kept"]
   dg = c5["slots"][gi].dg
   scale = serve.SERVED_SPECS[gi][0]
   qs = np.nonzero(st.graph == gi)[0]

@@ -71,7 +71,8 @@ def _serve_staggered(dg, g, roots, direction):
 @pytest.mark.gpu
 @pytest.mark.parametrize("scale,ef,seed,directed,direction", [
     (13, 16, 21, False, "beamer_lanes"), (12, 8, 22, True, "beamer_lanes"),
-    (12, 16, 23, False, "topdown"), (11, 16, 24, False, "bottomup"), (12, 8, 25, True, "tau")])
+    (12, 16, 23, False, "topdown"), (11, 16, 24, False, "bottomup"), (12, 8, 25, True, "tau"),
+    (13, 16, 26, False, "beamer"), (12, 8, 27, True, "beamer")])
 def test_engine_results_equal_alone_and_in_mix(scale, ef, seed, directed, direction):
   abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
   n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
@@ -111,3 +112,46 @@ def test_engine_hub_with_late_segments(direction):
   roots = np.concatenate([np.arange(hub_deg, hub_deg + 64), rng.integers(0, hub_deg, size=40)])
   with DeviceGraph.from_csr(g.n, g.out_ptr, g.out_idx, g.in_ptr, g.in_idx, g.symmetric) as dg:
     _serve_staggered(dg, g, roots, direction)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("admission", ["continuous", "waves"])
+def test_serve_stream_admissions_equal_oracle(admission):
+  """The serving loop over three resident graphs (two undirected R-MAT, one
+  directed) under a Poisson stream fast enough that queries queue: with
+  waves, an engine takes queries only while none is in flight (several
+  waves per graph), with continuous batching at every round.  Every query's
+  parents, levels, m and round count equal the oracle's."""
+  import torch
+  specs = [(10, 16, 31, False), (12, 8, 32, False), (11, 16, 33, True)]
+  dgs, graphs = [], []
+  for sc, ef, seed, directed in specs:
+    abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+    n, ip, ii, op, oi = gen_rmat.rmat_csr(sc, ef, seed, *abcd[:3], directed=directed)
+    graphs.append(O.Graph(n, ip, ii, op, oi, not directed))
+    dgs.append(DeviceGraph.from_rmat(sc, ef, seed, abcd=abcd, directed=directed))
+  slots = [serve.GraphSlot(d, 48, direction=serve.ADMISSION_DIRECTION[admission]) for d in dgs]
+  try:
+    nonisolated = [np.nonzero(np.diff(g.out_ptr) > 0)[0] for g in graphs]
+    stream = serve.poisson_stream(20000.0, 400, nonisolated, seed=7)
+    kept = {}
+
+    def keep(q, buf, m, rounds):
+      kept[q] = (buf[0].cpu().numpy(), buf[1].cpu().numpy(), m, rounds)
+
+    lat, m, _ = serve.serve_stream(slots, stream, deadline_s=300.0, on_complete=keep,
+                                   admission=admission)
+    torch.cuda.synchronize()
+    assert len(kept) == 400 and np.isfinite(lat).all()
+    for q in range(400):
+      g = graphs[int(stream.graph[q])]
+      lev0, par0, _, rounds0 = O.bfs(g, int(stream.root[q]))
+      par, lev, mq, rounds = kept[q]
+      np.testing.assert_array_equal(lev, lev0.astype(np.int16), err_msg=f"query {q}")
+      np.testing.assert_array_equal(par, par0, err_msg=f"query {q}")
+      assert (mq, rounds) == (O.m_edges(g, lev0), rounds0) == (int(m[q]), rounds), f"query {q}"
+  finally:
+    for s in slots:
+      s.close()
+    for d in dgs:
+      d.close()
