@@ -35,7 +35,10 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kHistCap = 4096;        // rounds recorded in the history arrays
 constexpr int64_t kBigDegree = 256;   // push: whole-grid flat expansion at/above this (default)
 constexpr int kBigLds = 4096;         // push: big-list prefix entries staged per pass
-constexpr int64_t kWaveDegree = 32;   // push: wavefront expansion at/above this (default)
+// push: a wavefront expands each vertex of out-degree at/above this alone
+// (default: none below kBigDegree; smaller rows are laid end to end)
+constexpr int64_t kWaveDegree = 256;
+constexpr int64_t kSpanCap = (int64_t)1 << 24;  // push: rows laid end to end stay below this
 #ifndef CG_PULL_HEAD
 #define CG_PULL_HEAD 2
 #endif
@@ -317,6 +320,40 @@ __device__ __forceinline__ void push_visit(const LevelArgs &A, int d, uint32_t u
   }
 }
 
+// Expands the rows of the calling wavefront's lanes (lane l: frontier
+// vertex u, out-edges [beg, beg + len), len < kSpanCap; 0 for none) laid
+// end to end: each step every lane takes one edge, finds its row (the last
+// row starting at or before it: row owners mark their starts in rm, a
+// prefix max fills the gaps, carried from step to step) and calls
+// visit(u of that row, head of the edge).  A wavefront thus spends steps on
+// its rows' edges, not on its longest row, and its lanes' visits are
+// independent of each other.  Wavefront-uniform; rm holds 64 entries.
+template <class Visit>
+__device__ __forceinline__ void push_rows_packed(const LevelArgs &A, int32_t *rm, uint32_t u,
+                                                 int64_t beg, int32_t len, Visit visit) {
+  const int lane = lane_id();
+  const int32_t incl = wave_scan_add(len);
+  const int32_t vstart = incl - len;
+  const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+  int rcarry = 0;
+  for (int32_t s0 = 0; s0 < total; s0 += kWave) {
+    rm[lane] = 0;
+    wave_sync();
+    const int32_t p = vstart - s0;
+    if (len && p >= 0 && p < kWave) rm[p] = lane + 1;
+    wave_sync();
+    const int r1 = max(wave_scan_max(rm[lane]), rcarry);
+    rcarry = __builtin_amdgcn_readlane(r1, kWave - 1);
+    const int32_t ve = s0 + lane;
+    const bool ok = ve < total;
+    const int r = ok ? r1 - 1 : 0;
+    const int32_t vs = __shfl(vstart, r);
+    const int64_t bs = __shfl(beg, r);
+    const uint32_t ur = (uint32_t)__shfl((int32_t)u, r);
+    if (ok) visit(ur, A.out_idx[bs + (ve - vs)]);
+  }
+}
+
 __device__ __forceinline__ void push_flush(PushLocal &L, unsigned long long *gtail, int32_t *gq,
                                            unsigned *lds_base) {
   __syncthreads();
@@ -354,24 +391,33 @@ __global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
   const int64_t wave = gtid / kWave;
   const int64_t nwaves = gstride / kWave;
   const int lane = lane_id();
-  for (int64_t base = wave * kWave; base < nq; base += nwaves * kWave) {
+  // Row map of the current step: rm[p] = 1 + the lane whose row starts at
+  // position p of the step (0: none).
+  __shared__ int32_t rmap[kWavesPerBlock][kWave];
+  int32_t *rm = rmap[threadIdx.x / kWave];
+  // Queue entries per window: 64, or fewer when 64 would leave wavefronts
+  // idle (a short queue), so every wavefront gets a share of the edges.
+  const int64_t per = (nq + nwaves - 1) / nwaves;
+  const int win = per >= kWave ? kWave : (per < 1 ? 1 : (int)per);
+  for (int64_t base = wave * win; base < nq; base += nwaves * win) {
     int64_t i = base + lane;
-    bool valid = i < nq;
+    bool valid = lane < win && i < nq;
     int32_t u = valid ? Q[i] : 0;
     int64_t beg = valid ? A.out_ptr[u] : 0;
     int64_t end = valid ? A.out_ptr[u + 1] : 0;
     int64_t deg = end - beg;
     bool big = valid && deg >= A.big_degree;
-    bool mid = valid && !big && deg >= A.wave_degree;
-    bool small = valid && deg < A.wave_degree;
+    // rows below kSpanCap keep the window's virtual range within int32
+    bool small = valid && !big && deg < A.wave_degree && deg < kSpanCap;
+    bool mid = valid && !big && !small;
     int64_t bslot = wave_append(&c->big_tail, big);
     if (big) A.bigq[bslot] = u;
-    if (small) {
-      for (int64_t e = beg; e < end; e++) push_visit(A, d, (uint32_t)u, A.out_idx[e], fnext, L, gtail, QN);
+    if (valid && !big) {
       L.scanned += (unsigned long long)deg;
+      L.rows++;
     }
-    if (mid) L.scanned += (unsigned long long)deg;
-    if (valid && !big) L.rows++;
+    push_rows_packed(A, rm, (uint32_t)u, beg, small ? (int32_t)deg : 0,
+                     [&](uint32_t pu, int32_t v) { push_visit(A, d, pu, v, fnext, L, gtail, QN); });
     uint64_t mm = ballot64(mid);
     while (mm) {
       int j = __ffsll((unsigned long long)mm) - 1;
@@ -843,9 +889,10 @@ __device__ __forceinline__ void push_visit_bm(const LevelArgs &A, int d, uint32_
 
 // One push round read from the frontier bitmap by the calling grid (every
 // thread of every block must call it): wavefronts take 64-vertex chunks of
-// the bitmap; a frontier vertex of out-degree below wave_degree is expanded
-// by its own lane, a larger one by the whole wavefront, 64 out-neighbours a
-// step.  Meant for graphs small enough that no whole-grid tier is needed.
+// the bitmap; the rows of a chunk's frontier vertices of out-degree below
+// wave_degree are laid end to end (push_rows_packed), a larger one is
+// expanded by the whole wavefront, 64 out-neighbours a step, and those of
+// out-degree >= big_degree by the whole grid after the next barrier.
 __device__ void push_level_bitmap(const LevelArgs &A, int d) {
   Ctrl *c = A.ctrl;
   __shared__ unsigned long long red[kWavesPerBlock];
@@ -858,6 +905,8 @@ __device__ void push_level_bitmap(const LevelArgs &A, int d) {
   const int lane = lane_id();
   const int64_t wave = gtid / kWave, nwaves = gstride / kWave;
   const int64_t nchunks = A.nwords / 2;
+  __shared__ int32_t rmap[kWavesPerBlock][kWave];
+  int32_t *rm = rmap[threadIdx.x / kWave];
   unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0, found = 0;
   for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
     const uint32_t word = fcur[ch * 2 + (lane >> 5)];
@@ -865,8 +914,8 @@ __device__ void push_level_bitmap(const LevelArgs &A, int d) {
     const bool on = v < A.n && ((word >> (lane & 31)) & 1u);
     if (!__any(on)) continue;
     const int64_t beg = on ? A.out_ptr[v] : 0, end = on ? A.out_ptr[v + 1] : 0;
-    const bool small = on && end - beg < A.wave_degree;
     const bool big = on && end - beg >= A.big_degree;
+    const bool small = on && !big && end - beg < A.wave_degree && end - beg < kSpanCap;
     if (on && !big) {
       rows++;
       scanned += (unsigned long long)(end - beg);
@@ -874,9 +923,8 @@ __device__ void push_level_bitmap(const LevelArgs &A, int d) {
     // the whole grid expands the big ones after the next barrier
     const int64_t slot = wave_append(&c->big_tail, big);
     if (big) A.bigq[slot] = (int32_t)v;
-    if (small)
-      for (int64_t e = beg; e < end; e++)
-        push_visit_bm(A, d, (uint32_t)v, A.out_idx[e], fnext, mf, in_sum, found);
+    push_rows_packed(A, rm, (uint32_t)v, beg, small ? (int32_t)(end - beg) : 0,
+                     [&](uint32_t pu, int32_t w) { push_visit_bm(A, d, pu, w, fnext, mf, in_sum, found); });
     uint64_t wide = ballot64(on && !small && !big);
     while (wide) {
       const int j = __ffsll((unsigned long long)wide) - 1;

@@ -85,7 +85,8 @@ typedef struct {
   int32_t seg_grab;   /* multi-source: 256-entry hub-row segments a wavefront
                          takes per grab, 1..1024                       (32)  */
   int32_t push_wave_degree; /* single root: a wavefront expands each frontier
-                         vertex of out-degree >= this, 1..2^30          (32)  */
+                         vertex of out-degree >= this alone (smaller rows
+                         are laid end to end, 64 edges a step), 1..2^30 (256) */
   int32_t push_big_degree;  /* single root: the whole grid expands frontier
                          vertices of out-degree >= this, >= push_wave_degree
                          and <= 2^30                                    (256) */
