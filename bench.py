@@ -72,18 +72,18 @@ def level_bytes(n, direction, edges, rows, nf, frontier):
   """Algorithmic HBM bytes of one single-root level.  Pull: visited and
   frontier bitmaps (V/8 each), the offsets of every row opened (8 U), 4 B
   per adjacency entry read (4 E) and parent + level of each new vertex
-  (8 N).  Push: the frontier queue (4 F), its offsets (8 F), 4 E, 8 N and
-  the next queue (4 N)."""
+  (8 N).  Push: the frontier bitmap (V/8), the frontier's offsets (8 F),
+  4 E and 8 N."""
   if direction == 1:
     return n / 8 + n / 8 + 8 * rows + 4 * edges + 8 * nf
-  return 12 * frontier + 4 * edges + 12 * nf
+  return n / 8 + 8 * frontier + 4 * edges + 8 * nf
 
 
 def per_root_roofline(n, results):
   """The dominant level of each single-root search (its longest one, a pull
   level on these graphs) over all searches: algorithmic bytes from the
-  level's device counters / the level's event time (its traversal kernels,
-  bfs_push_round .. bfs_pull_round)."""
+  level's device counters / the level's event time (its kernels bfs_level,
+  bfs_push_scan and bfs_push_big)."""
   byts, secs, dirs = 0.0, 0.0, []
   for r in results:
     st = r.stats
@@ -95,7 +95,7 @@ def per_root_roofline(n, results):
     dirs.append(int(st.direction_history[d]))
   achieved = byts / secs / 1e9 if secs > 0 else 0.0
   return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-          "frac": round(achieved / PEAK_HBM_GBS, 4), "kernel": "bfs_pull_round",
+          "frac": round(achieved / PEAK_HBM_GBS, 4), "kernel": "bfs_level",
           "level": "longest level of each search",
           "pull_levels": int(sum(dirs)), "searches": len(results),
           "avg_level_ms": round(secs * 1e3 / max(len(results), 1), 4),

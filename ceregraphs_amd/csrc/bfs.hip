@@ -2,24 +2,26 @@
 //
 // Replaces the reference's on-device round loop f_spmv_iter
 // (bfs/implementation/src/bool_pe.csl:673-1161).  One round = one BFS level:
-//   * push round (compute_topdown, bool_pe.csl:865-887): the level-d queue is
-//     expanded over out-neighbours; a vertex is claimed by atomicOr on the
-//     visited bitmap (replacing the row OR-reduce, collectives_2d reduce_or),
-//     and every frontier predecessor of a vertex new at this level offers
-//     itself with atomicMin, so the parent is the lowest-id predecessor;
+//   * push round (compute_topdown, bool_pe.csl:865-887): the vertices of the
+//     level-d frontier bitmap are expanded over out-neighbours; a vertex is
+//     claimed by atomicOr on the visited bitmap (replacing the row OR-reduce,
+//     collectives_2d reduce_or), and every frontier predecessor of a vertex
+//     new at this level offers itself with atomicMin, so the parent is the
+//     lowest-id predecessor;
 //   * pull round (compute_bottomup, bool_pe.csl:899-921): every unvisited
 //     vertex scans its ascending in-neighbours and stops at the first one in
 //     the level-d frontier bitmap -- the first hit is the lowest id;
 //   * the decide step (reduce_done + the termination relay + the switch,
 //     bool_pe.csl:947-1156) reads device counters and picks the next
 //     direction or ends the search, with no host round trip.
-// The frontier lives in HBM both as a queue (push) and as a bitmap (pull).
-// Both are produced by every round, so either direction can follow either.
-//
-// Buffers are rotated by level: queue Q[d&1], bitmap F[d%3].  Round d reads
-// Q[d&1]/F[d%3], writes Q[(d+1)&1]/F[(d+1)%3] and zeroes F[(d+2)%3] (the
-// level d-1 bitmap, no longer read), so every bitmap is clear one round
-// before it is written.
+// The frontier lives in HBM as a bitmap that both directions read and
+// write, so either direction can follow either.  Frontier bitmaps are
+// rotated by level: round d reads F[d%3], writes F[(d+1)%3] and zeroes
+// F[(d+2)%3] (the level d-1 bitmap, no longer read), so every bitmap is
+// clear one round before it is written.  A level of the per-level loop is
+// four launches: bfs_level (the push or the pull), bfs_push_scan and
+// bfs_push_big (the push's vertices of out-degree >= big_degree, expanded
+// by the whole grid) and bfs_decide.
 #include <hip/hip_cooperative_groups.h>
 
 #include <cstring>
@@ -52,10 +54,8 @@ constexpr int kPass = kPassSteps * kWave;  // pull: tail entries per pass
 #define CG_PULL_GRAB 16
 #endif
 constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the cursors (default)
-constexpr int kLdsQueue = 2048;       // per-block staging for claimed vertices
 
 struct Ctrl {
-  unsigned long long qtail[2];  // queue lengths, Q[0] and Q[1]
   unsigned long long big_tail;  // push: count of whole-grid vertices
   unsigned long long big_edges; // push: their total degree
   unsigned long long nf;        // vertices discovered this round
@@ -69,7 +69,6 @@ struct Ctrl {
   long long rounds;
   int dir;   // direction of the current round: 0 push, 1 pull
   int light; // the current round's pull splits its chunks statically
-  int listed; // the current round's frontier is in its queue (a pull leaves only the bitmap)
   int done;
   int status;
   unsigned long long t0;
@@ -97,7 +96,6 @@ struct Workspace {
   int64_t nwords = 0;  // u32 words per bitmap, rounded up to 64 vertices
   uint32_t *visited = nullptr;
   uint32_t *fb[3] = {nullptr, nullptr, nullptr};
-  int32_t *q[2] = {nullptr, nullptr};
   int32_t *bigq = nullptr;
   int64_t *bigpre = nullptr;  // exclusive degree prefix of bigq
   int32_t *levels = nullptr;   // internal outputs (host-output calls)
@@ -115,7 +113,7 @@ static constexpr size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 // are host memory (copied out at the end).
 static constexpr size_t ws1_size(int64_t n, bool internal_out) {
   return align_up(sizeof(uint32_t) * (size_t)(((n + 63) / 64) * 2)) * 4 +  // visited, fb[3]
-         align_up(sizeof(int32_t) * (size_t)n) * 3 +                       // q[2], bigq
+         align_up(sizeof(int32_t) * (size_t)n) +                           // bigq
          align_up(sizeof(int64_t) * (size_t)(n + 1)) +                      // bigpre
          (internal_out ? align_up(sizeof(int32_t) * (size_t)n) * 2 : 0) +   // levels, parents
          align_up(sizeof(Ctrl)) + align_up(sizeof(Hist)) + align_up(64) +
@@ -126,13 +124,11 @@ static constexpr size_t ws1_size(int64_t n, bool internal_out) {
 static void ws1_layout(int64_t n, bool internal_out, char *p, Workspace *w) {
   const int64_t nwords = ((n + 63) / 64) * 2;
   const size_t bm = align_up(sizeof(uint32_t) * nwords);
-  const size_t qb = align_up(sizeof(int32_t) * n);
   w->n = n;
   w->nwords = nwords;
   w->visited = (uint32_t *)p; p += bm;
   for (int i = 0; i < 3; i++) { w->fb[i] = (uint32_t *)p; p += bm; }
-  for (int i = 0; i < 2; i++) { w->q[i] = (int32_t *)p; p += qb; }
-  w->bigq = (int32_t *)p; p += qb;
+  w->bigq = (int32_t *)p; p += align_up(sizeof(int32_t) * n);
   w->bigpre = (int64_t *)p; p += align_up(sizeof(int64_t) * (n + 1));
   if (internal_out) {
     w->levels = (int32_t *)p; p += align_up(sizeof(int32_t) * n);
@@ -169,7 +165,7 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
 // hipGetSymbolAddress, once per device, with that device current.
 constexpr int64_t kNoAllocMaxN = (int64_t)1 << 21;
 constexpr size_t kNoAllocBytes = ws1_size(kNoAllocMaxN, false);
-static_assert(kNoAllocBytes == 43148032, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
+static_assert(kNoAllocBytes == 26370816, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 constexpr int kMaxDevices = 64;
@@ -217,7 +213,6 @@ struct LevelArgs {
   uint32_t *parents;
   uint32_t *visited;
   uint32_t *fb[3];
-  int32_t *q[2];
   int32_t *bigq;
   int64_t *bigpre;
   Ctrl *ctrl;
@@ -255,10 +250,7 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, BfsParams P, long long t
   A.parents[s] = (uint32_t)s;
   A.visited[s >> 5] |= 1u << (s & 31);
   A.fb[0][s >> 5] |= 1u << (s & 31);
-  A.q[0][0] = (int32_t)s;
   Ctrl *c = A.ctrl;
-  st_ctrl(&c->qtail[0], 1ull);
-  st_ctrl(&c->qtail[1], 0ull);
   st_ctrl(&c->big_tail, 0ull);
   st_ctrl(&c->nf, 0ull);
   st_ctrl(&c->mf, 0ull);
@@ -269,7 +261,6 @@ __global__ void bfs_init_source(LevelArgs A, int64_t s, BfsParams P, long long t
   st_ctrl(&c->prev_nf, 1ll);
   st_ctrl(&c->seen, 1ll);
   st_ctrl(&c->light, 0);
-  st_ctrl(&c->listed, 1);
   st_ctrl(&c->rounds, 0ll);
   // round 0 is push (bool_pe.csl:714-719) unless every round pulls
   st_ctrl(&c->dir, P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0);
@@ -290,32 +281,28 @@ __global__ void bfs_start(Ctrl *c) {
 // ---------------------------------------------------------------------------
 // Push round
 // ---------------------------------------------------------------------------
-struct PushLocal {
-  int32_t *lq;                // LDS queue
-  unsigned *lq_n;             // LDS count
-  unsigned long long mf, in_sum, scanned, rows;
-};
-
-// Offers frontier vertex u as predecessor of v in round d.
-__device__ __forceinline__ void push_visit(const LevelArgs &A, int d, uint32_t u, int32_t v,
-                                           uint32_t *fnext, PushLocal &L,
-                                           unsigned long long *gtail, int32_t *gq) {
+// Offers frontier vertex u as predecessor of v in round d: the first to set
+// v's visited bit claims it (level, next-frontier bit, counts), and every
+// frontier in-neighbour offers itself as v's parent with atomicMin, so the
+// lowest id wins.  A push keeps no queue: the next frontier is the bitmap.
+__device__ __forceinline__ void push_visit_bm(const LevelArgs &A, int d, uint32_t u, int32_t v,
+                                              uint32_t *fnext, unsigned long long &mf,
+                                              unsigned long long &in_sum,
+                                              unsigned long long &found) {
   const uint32_t w = (uint32_t)v >> 5, b = 1u << (v & 31);
-  uint32_t vw = __builtin_nontemporal_load(&A.visited[w]);
+  const uint32_t vw = __builtin_nontemporal_load(&A.visited[w]);
   if (!(vw & b)) {
-    uint32_t old = atomicOr(&A.visited[w], b);
+    const uint32_t old = atomicOr(&A.visited[w], b);
     if (!(old & b)) {
       A.levels[v] = d + 1;
       atomicOr(&fnext[w], b);
-      unsigned slot = atomicAdd(L.lq_n, 1u);
-      if (slot < (unsigned)kLdsQueue) L.lq[slot] = v;
-      else gq[atomicAdd(gtail, 1ull)] = v;
-      L.mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
-      L.in_sum += (unsigned long long)(A.in_ptr[v + 1] - A.in_ptr[v]);
+      found++;
+      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
+      in_sum += (unsigned long long)(A.in_ptr[v + 1] - A.in_ptr[v]);
     }
     atomicMin(&A.parents[v], u);
   } else {
-    int lv = __hip_atomic_load(&A.levels[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int lv = __hip_atomic_load(&A.levels[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (lv == d + 1 || lv == -1) atomicMin(&A.parents[v], u);
   }
 }
@@ -352,113 +339,6 @@ __device__ __forceinline__ void push_rows_packed(const LevelArgs &A, int32_t *rm
     const uint32_t ur = (uint32_t)__shfl((int32_t)u, r);
     if (ok) visit(ur, A.out_idx[bs + (ve - vs)]);
   }
-}
-
-__device__ __forceinline__ void push_flush(PushLocal &L, unsigned long long *gtail, int32_t *gq,
-                                           unsigned *lds_base) {
-  __syncthreads();
-  unsigned cnt = *L.lq_n;
-  if (cnt > (unsigned)kLdsQueue) cnt = kLdsQueue;
-  if (threadIdx.x == 0) *lds_base = (unsigned)(cnt ? atomicAdd(gtail, (unsigned long long)cnt) : 0);
-  __syncthreads();
-  unsigned long long base = *lds_base;
-  for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x) gq[base + i] = L.lq[i];
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(kBlock) bfs_push_round(LevelArgs A, int d) {
-  Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
-  __shared__ int32_t lq[kLdsQueue];
-  __shared__ unsigned lq_n;
-  __shared__ unsigned lds_base;
-  __shared__ unsigned long long red[kWavesPerBlock];
-  if (threadIdx.x == 0) lq_n = 0;
-  __syncthreads();
-  PushLocal L{lq, &lq_n, 0, 0, 0, 0};
-  const int cur = d & 1, nxt = (d + 1) & 1;
-  const int32_t *Q = A.q[cur];
-  int32_t *QN = A.q[nxt];
-  unsigned long long *gtail = &c->qtail[nxt];
-  uint32_t *fnext = A.fb[(d + 1) % 3];
-  uint32_t *fclear = A.fb[(d + 2) % 3];
-  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[cur]);
-
-  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t gstride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t w = gtid; w < A.nwords; w += gstride) fclear[w] = 0;
-
-  const int64_t wave = gtid / kWave;
-  const int64_t nwaves = gstride / kWave;
-  const int lane = lane_id();
-  // Row map of the current step: rm[p] = 1 + the lane whose row starts at
-  // position p of the step (0: none).
-  __shared__ int32_t rmap[kWavesPerBlock][kWave];
-  int32_t *rm = rmap[threadIdx.x / kWave];
-  // Queue entries per window: 64, or fewer when 64 would leave wavefronts
-  // idle (a short queue), so every wavefront gets a share of the edges.
-  const int64_t per = (nq + nwaves - 1) / nwaves;
-  const int win = per >= kWave ? kWave : (per < 1 ? 1 : (int)per);
-  for (int64_t base = wave * win; base < nq; base += nwaves * win) {
-    int64_t i = base + lane;
-    bool valid = lane < win && i < nq;
-    int32_t u = valid ? Q[i] : 0;
-    int64_t beg = valid ? A.out_ptr[u] : 0;
-    int64_t end = valid ? A.out_ptr[u + 1] : 0;
-    int64_t deg = end - beg;
-    bool big = valid && deg >= A.big_degree;
-    // rows below kSpanCap keep the window's virtual range within int32
-    bool small = valid && !big && deg < A.wave_degree && deg < kSpanCap;
-    bool mid = valid && !big && !small;
-    int64_t bslot = wave_append(&c->big_tail, big);
-    if (big) A.bigq[bslot] = u;
-    if (valid && !big) {
-      L.scanned += (unsigned long long)deg;
-      L.rows++;
-    }
-    push_rows_packed(A, rm, (uint32_t)u, beg, small ? (int32_t)deg : 0,
-                     [&](uint32_t pu, int32_t v) { push_visit(A, d, pu, v, fnext, L, gtail, QN); });
-    uint64_t mm = ballot64(mid);
-    while (mm) {
-      int j = __ffsll((unsigned long long)mm) - 1;
-      mm &= mm - 1;
-      int32_t uj = __shfl(u, j);
-      int64_t bj = __shfl(beg, j), ej = __shfl(end, j);
-      for (int64_t e = bj + lane; e < ej; e += kWave) push_visit(A, d, (uint32_t)uj, A.out_idx[e], fnext, L, gtail, QN);
-    }
-  }
-  push_flush(L, gtail, QN, &lds_base);
-  unsigned long long s;
-  s = block_sum(L.mf, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
-  s = block_sum(L.in_sum, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
-  s = block_sum(L.scanned, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
-  s = block_sum(L.rows, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
-}
-
-// Lists the frontier bitmap of round d in its queue when a push round
-// follows a pull round (a pull leaves only the bitmap).  Words are read in
-// order and appended through a block-staged queue, one global atomic per
-// block; the frontier is small when a pull hands over to a push.
-__global__ void __launch_bounds__(kBlock) bfs_compact(LevelArgs A, int d) {
-  Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0 || ld_ctrl(&c->listed)) return;
-  const uint32_t *fcur = A.fb[d % 3];
-  __shared__ int32_t lq[kLdsQueue];
-  __shared__ unsigned lq_n;
-  __shared__ unsigned long long lq_base;
-  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kLdsQueue, &c->qtail[d & 1], A.q[d & 1]};
-  sq.init();
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t nround = ((A.n + stride - 1) / stride) * stride;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nround; v += stride) {
-    const bool on = v < A.n && ((fcur[v >> 5] >> (v & 31)) & 1u);
-    sq.push(on, (int32_t)v);
-  }
-  sq.flush();
 }
 
 // Degree prefix of the push round's big list (one block): bigpre[j] = sum of
@@ -514,17 +394,9 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
   if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
   const int64_t nb = (int64_t)ld_ctrl(&c->big_tail);
   if (nb == 0) return;
-  __shared__ int32_t lq[kLdsQueue];
-  __shared__ unsigned lq_n;
-  __shared__ unsigned lds_base;
   __shared__ unsigned long long red[kWavesPerBlock];
   __shared__ long long pre[kBigLds + 1];
-  if (threadIdx.x == 0) lq_n = 0;
-  __syncthreads();
-  PushLocal L{lq, &lq_n, 0, 0, 0, 0};
-  const int nxt = (d + 1) & 1;
-  int32_t *QN = A.q[nxt];
-  unsigned long long *gtail = &c->qtail[nxt];
+  unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0, found = 0;
   uint32_t *fnext = A.fb[(d + 1) % 3];
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gstride = (int64_t)gridDim.x * blockDim.x;
@@ -542,23 +414,24 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
       }
       int32_t u = A.bigq[j0 + a];
       int64_t e = A.out_ptr[u] + (f - pre[a]);
-      push_visit(A, d, (uint32_t)u, A.out_idx[e], fnext, L, gtail, QN);
+      push_visit_bm(A, d, (uint32_t)u, A.out_idx[e], fnext, mf, in_sum, found);
     }
     if (gtid == 0) {
-      L.scanned += (unsigned long long)(hi - lo);
-      L.rows += (unsigned long long)cnt;
+      scanned += (unsigned long long)(hi - lo);
+      rows += (unsigned long long)cnt;
     }
     __syncthreads();
   }
-  push_flush(L, gtail, QN, &lds_base);
   unsigned long long s;
-  s = block_sum(L.mf, red);
+  s = block_sum(found, red);
+  if (threadIdx.x == 0 && s) atomicAdd(&c->nf, s);
+  s = block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
-  s = block_sum(L.in_sum, red);
+  s = block_sum(in_sum, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->in_sum, s);
-  s = block_sum(L.scanned, red);
+  s = block_sum(scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
-  s = block_sum(L.rows, red);
+  s = block_sum(rows, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
@@ -595,8 +468,8 @@ __device__ __forceinline__ void pull_prefetch(const LevelArgs &A, int32_t ch, bo
 }
 
 // One pull round over the whole graph by the calling grid (every thread of
-// every block must call it): the body of bfs_pull_round, and of a pull level
-// of the persistent search.
+// every block must call it): a pull level of bfs_level or of the persistent
+// search.
 __device__ void pull_level(const LevelArgs &A, int d) {
   Ctrl *c = A.ctrl;
   __shared__ unsigned long long red[kWavesPerBlock];
@@ -740,10 +613,9 @@ __device__ void pull_level(const LevelArgs &A, int d) {
       in_sum += (unsigned long long)(end - beg);
       found++;
     }
-    // The new frontier is the bitmap alone: a pull round is followed by a
-    // pull far more often than by a push, which lists the bitmap first
-    // (bfs_compact).  Appending every found vertex to a queue here made the
-    // dense level wait on one shared tail counter.
+    // The new frontier is the bitmap alone, which a push reads directly.
+    // Appending every found vertex to a queue here made the dense level wait
+    // on one shared tail counter.
     uint64_t hm = ballot64(hit);
     if (hm && lane == 0) {
       uint32_t lo = (uint32_t)hm, hi = (uint32_t)(hm >> 32);
@@ -781,10 +653,6 @@ __device__ void pull_level(const LevelArgs &A, int d) {
   if (threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
-__global__ void __launch_bounds__(kBlock, 8) bfs_pull_round(LevelArgs A, int d) {
-  if (ld_ctrl(&A.ctrl->done) || ld_ctrl(&A.ctrl->dir) != 1) return;
-  pull_level(A, d);
-}
 
 // ---------------------------------------------------------------------------
 // Decide: termination, direction, history (one thread)
@@ -808,7 +676,6 @@ __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsPara
   }
   st_ctrl(&c->rounds, (long long)(d + 1));
   st_ctrl(&c->t_end, now);
-  st_ctrl(&c->qtail[d & 1], 0ull);
   st_ctrl(&c->big_tail, 0ull);
   st_ctrl(&c->mu, mu);
   st_ctrl(&c->nf, 0ull);
@@ -836,7 +703,6 @@ __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsPara
       }
   }
   st_ctrl(&c->prev_nf, nf);
-  st_ctrl(&c->listed, cur == 0 ? 1 : 0);
   // The next pull visits every chunk; once fewer vertices with in-edges are
   // unreached than there are chunks, most chunks have nothing to do.
   const long long seen = ld_ctrl(&c->seen) + nf;
@@ -853,39 +719,12 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Ctrl *c = A.ctrl;
   if (ld_ctrl(&c->done)) return;
-  // a push round listed its new vertices, a pull round counted them
-  const long long nf = ld_ctrl(&c->dir) == 0 ? (long long)ld_ctrl(&c->qtail[(d + 1) & 1])
-                                             : (long long)ld_ctrl(&c->nf);
-  decide_round(A, d, n, P, nf);
+  decide_round(A, d, n, P, (long long)ld_ctrl(&c->nf));
 }
 
 // ---------------------------------------------------------------------------
 // Persistent search: every level in one cooperative launch
 // ---------------------------------------------------------------------------
-
-// Offers frontier vertex u as predecessor of v in round d, as push_visit,
-// for a push that keeps no queue: the new frontier is the bitmap alone.
-__device__ __forceinline__ void push_visit_bm(const LevelArgs &A, int d, uint32_t u, int32_t v,
-                                              uint32_t *fnext, unsigned long long &mf,
-                                              unsigned long long &in_sum,
-                                              unsigned long long &found) {
-  const uint32_t w = (uint32_t)v >> 5, b = 1u << (v & 31);
-  const uint32_t vw = __builtin_nontemporal_load(&A.visited[w]);
-  if (!(vw & b)) {
-    const uint32_t old = atomicOr(&A.visited[w], b);
-    if (!(old & b)) {
-      A.levels[v] = d + 1;
-      atomicOr(&fnext[w], b);
-      found++;
-      mf += (unsigned long long)(A.out_ptr[v + 1] - A.out_ptr[v]);
-      in_sum += (unsigned long long)(A.in_ptr[v + 1] - A.in_ptr[v]);
-    }
-    atomicMin(&A.parents[v], u);
-  } else {
-    const int lv = __hip_atomic_load(&A.levels[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (lv == d + 1 || lv == -1) atomicMin(&A.parents[v], u);
-  }
-}
 
 // One push round read from the frontier bitmap by the calling grid (every
 // thread of every block must call it): wavefronts take 64-vertex chunks of
@@ -976,6 +815,17 @@ __device__ void push_big_rows(const LevelArgs &A, int d) {
   s = block_sum(scanned, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
   if (gtid == 0 && nb) atomicAdd(&c->rows, (unsigned long long)nb);
+}
+
+// One level of the per-level loop: the push (read from the frontier bitmap;
+// out-degree >= big_degree left to bfs_push_big) or the pull, as the
+// previous decide chose.  One launch serves both, so a level costs one
+// dispatch less than a kernel per direction.
+__global__ void __launch_bounds__(kBlock, 8) bfs_level(LevelArgs A, int d) {
+  const Ctrl *c = A.ctrl;
+  if (ld_ctrl(&c->done)) return;
+  if (ld_ctrl(&c->dir) == 0) push_level_bitmap(A, d);
+  else pull_level(A, d);
 }
 
 // The whole level loop in one launch (cooperative: every block is resident,
@@ -1121,8 +971,6 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
   A.parents = par;
   A.visited = ws.visited;
   for (int i = 0; i < 3; i++) A.fb[i] = ws.fb[i];
-  A.q[0] = ws.q[0];
-  A.q[1] = ws.q[1];
   A.bigq = ws.bigq;
   A.bigpre = ws.bigpre;
   A.ctrl = ws.ctrl;
@@ -1183,11 +1031,9 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
           rev[nrev++] = rb;
           hipEventRecord(ra, st);
         }
-        bfs_compact<<<blocks, kBlock, 0, st>>>(A, (int)d);
-        bfs_push_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
+        bfs_level<<<blocks, kBlock, 0, st>>>(A, (int)d);
         bfs_push_scan<<<1, 1024, 0, st>>>(A);
         bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);
-        bfs_pull_round<<<blocks, kBlock, 0, st>>>(A, (int)d);
         if (rb) hipEventRecord(rb, st);
         bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n, P);
       }
