@@ -55,22 +55,30 @@ constexpr int kPass = kPassSteps * kWave;  // pull: tail entries per pass
 #endif
 constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the cursors (default)
 
+// The round counters each take a 128-byte line of their own: every block
+// of a round adds to them once at its end, and device-scope atomics on one
+// line are serialised, with each other and with the flag reads every block
+// of the next kernel starts with.
+#ifndef CG_CTRL_LINE
+#define CG_CTRL_LINE 128
+#endif
 struct Ctrl {
-  unsigned long long big_tail;  // push: count of whole-grid vertices
-  unsigned long long big_edges; // push: their total degree
-  unsigned long long nf;        // vertices discovered this round
-  unsigned long long mf;        // out-degree sum of the discovered vertices
-  unsigned long long in_sum;    // in-degree sum of the discovered vertices
-  unsigned long long scanned;   // adjacency entries read this round
-  unsigned long long rows;      // adjacency lists opened this round
+  alignas(CG_CTRL_LINE) unsigned long long big_tail;  // push: count of whole-grid vertices
+  alignas(CG_CTRL_LINE) unsigned long long nf;        // vertices discovered this round
+  alignas(CG_CTRL_LINE) unsigned long long mf;        // out-degree sum of the discovered vertices
+  alignas(CG_CTRL_LINE) unsigned long long in_sum;    // in-degree sum of the discovered vertices
+  alignas(CG_CTRL_LINE) unsigned long long scanned;   // adjacency entries read this round
+  alignas(CG_CTRL_LINE) unsigned long long rows;      // adjacency lists opened this round
+  // read by every block of a round, written by the decide
+  alignas(CG_CTRL_LINE) int dir;   // direction of the current round: 0 push, 1 pull
+  int light; // the current round's pull splits its chunks statically
+  int done;
+  int status;
+  unsigned long long big_edges; // push: the big vertices' total degree
   long long mu;                 // in-edges of still-unvisited vertices
   long long prev_nf;
   long long seen;               // vertices reached so far
   long long rounds;
-  int dir;   // direction of the current round: 0 push, 1 pull
-  int light; // the current round's pull splits its chunks statically
-  int done;
-  int status;
   unsigned long long t0;
   unsigned long long t_end;
 };
@@ -165,7 +173,7 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
 // hipGetSymbolAddress, once per device, with that device current.
 constexpr int64_t kNoAllocMaxN = (int64_t)1 << 21;
 constexpr size_t kNoAllocBytes = ws1_size(kNoAllocMaxN, false);
-static_assert(kNoAllocBytes == 26370816, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
+static_assert(kNoAllocBytes == 26371584, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 constexpr int kMaxDevices = 64;
