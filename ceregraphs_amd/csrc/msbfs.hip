@@ -81,18 +81,30 @@ int check_options(const cgbfs_options *o);
 // Round counters come in two parts: the push part (nf .. pairs) and the
 // pull part (pnf .. ppairs).  nf counts only vertices the pull did not
 // reach, so nf + pnf is the number of vertices with a new lane.
+// A counter on a 128-byte line of its own.
+struct alignas(128) LineCtr {
+  unsigned long long v;
+};
+
+// The round counters and tails each take a 128-byte line: every block of a
+// round kernel adds to them (or appends through them) once, and device-scope
+// atomics on one line are serialised, with each other and with the reads of
+// the read-mostly fields (the last line) that every block starts with.
 struct MsCtrl {
-  unsigned long long qtail[2];
-  unsigned long long nf;
-  unsigned long long scanned;  // adjacency entries read this round
-  unsigned long long rows;     // adjacency lists opened this round
-  unsigned long long pairs;    // (lane, vertex) pairs reached this round
-  unsigned long long pnf, pscanned, prows, ppairs;
-  unsigned long long mf;       // out-degree sum of the vertices with a new lane
-  unsigned long long live;     // lanes that reached a vertex this round
-  unsigned long long segtail[2];  // active segments this round: pull, push
-  unsigned long long segtail_late;  // active pull segments past their vertex's first run
-  unsigned long long pmask;    // lanes the next round pulls
+  LineCtr qtail[2];
+  alignas(128) unsigned long long nf;
+  alignas(128) unsigned long long scanned;  // adjacency entries read this round
+  alignas(128) unsigned long long rows;     // adjacency lists opened this round
+  alignas(128) unsigned long long pairs;    // (lane, vertex) pairs reached this round
+  alignas(128) unsigned long long pnf;
+  alignas(128) unsigned long long pscanned;
+  alignas(128) unsigned long long prows;
+  alignas(128) unsigned long long ppairs;
+  alignas(128) unsigned long long mf;       // out-degree sum of the vertices with a new lane
+  alignas(128) unsigned long long live;     // lanes that reached a vertex this round
+  LineCtr segtail[2];  // active segments this round: pull, push
+  alignas(128) unsigned long long segtail_late;  // active pull segments past their vertex's first run
+  alignas(128) unsigned long long pmask;    // lanes the next round pulls
   unsigned long long qmask;    // lanes the next round pushes
   long long prev_nf;
   long long rounds;            // the round the next kernels work on
@@ -792,8 +804,8 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
   const unsigned long long deg = s_deg;  // out-degree sum of the distinct seed vertices
   if (l == 0) {
     const unsigned long long nq = s_nq;
-    st_ctrl(&c->qtail[0], nq);
-    st_ctrl(&c->qtail[1], 0ull);
+    st_ctrl(&c->qtail[0].v, nq);
+    st_ctrl(&c->qtail[1].v, 0ull);
     st_ctrl(&c->nf, 0ull);
     st_ctrl(&c->scanned, 0ull);
     st_ctrl(&c->rows, 0ull);
@@ -844,10 +856,10 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->gdir, g);
     st_ctrl(&c->mf, 0ull);
     st_ctrl(&c->live, 0ull);
-    st_ctrl(&c->segtail[0], 0ull);
-    st_ctrl(&c->segtail[1], 0ull);
+    st_ctrl(&c->segtail[0].v, 0ull);
+    st_ctrl(&c->segtail[1].v, 0ull);
     st_ctrl(&c->segtail_late, 0ull);
-    const long long nq0 = (long long)ld_ctrl(&c->qtail[0]);
+    const long long nq0 = (long long)ld_ctrl(&c->qtail[0].v);
     st_ctrl(&c->prev_nf, nq0);
     st_ctrl(&c->listed, 1);
     st_ctrl(&c->use_fb, nq0 * 8 <= (long long)A.n ? 1 : 0);
@@ -1125,7 +1137,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A, int late) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
   if (!R.pm) return;
-  const int32_t nact = (int32_t)ld_ctrl(late ? &c->segtail_late : &c->segtail[0]);
+  const int32_t nact = (int32_t)ld_ctrl(late ? &c->segtail_late : &c->segtail[0].v);
   if (nact == 0) return;  // uniform: written by an earlier kernel
   __shared__ unsigned long long red[kMsWaves];
   const uint64_t *fr = A.fw[R.d & 1];
@@ -1176,7 +1188,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_compact(MsArgs A, const int32_t *
   __shared__ int32_t lq[kMsLdsQueue];
   __shared__ unsigned lq_n;
   __shared__ unsigned long long lq_base;
-  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kMsLdsQueue, &c->qtail[d & 1], A.q[d & 1]};
+  StagedQueue sq{lq, &lq_n, &lq_base, (unsigned)kMsLdsQueue, &c->qtail[d & 1].v, A.q[d & 1]};
   sq.init();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t nround = ((A.n + stride - 1) / stride) * stride;
@@ -1197,7 +1209,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int which) 
   const int d = R.d;
   const SegList &L = which ? A.pout : A.pin;
   const uint64_t *fr = A.fw[d & 1];
-  unsigned long long *tail = &A.ctrl->segtail[which];
+  unsigned long long *tail = &A.ctrl->segtail[which].v;
   int32_t *out = A.segq[which];
   const int lane = lane_id();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1290,8 +1302,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
   uint64_t *nx = A.fw[(d + 1) & 1];
   const int32_t *Q = A.q[d & 1];
   int32_t *QN = A.q[(d + 1) & 1];
-  unsigned long long *tail = &c->qtail[(d + 1) & 1];
-  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1]);
+  unsigned long long *tail = &c->qtail[(d + 1) & 1].v;
+  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1].v);
   const int lane = lane_id();
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t wave = gtid / kWave, nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
@@ -1303,7 +1315,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
   if (phase == 0) sq.init();
   // Work items: queue entries (skipping big vertices), then the segments of
   // the big frontier vertices.  Both loops are wavefront-uniform.
-  const int64_t nitems = nq + (int64_t)ld_ctrl(&c->segtail[1]);
+  const int64_t nitems = nq + (int64_t)ld_ctrl(&c->segtail[1].v);
   for (int64_t it = wave; it < nitems; it += nwaves) {
     int32_t u;
     int64_t beg, end;
@@ -1374,7 +1386,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
   LaneTally tally;
   const uint64_t *nx = A.fw[(d + 1) & 1];
   const int32_t *QN = A.q[(d + 1) & 1];
-  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[(d + 1) & 1]);
+  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[(d + 1) & 1].v);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long nf = 0, pairs = 0, mf = 0;
@@ -1427,7 +1439,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_clear(MsArgs A) {
   const int d = R.d;
   uint64_t *fr = A.fw[d & 1];
   const int32_t *Q = A.q[d & 1];
-  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1]);
+  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1].v);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
        i += (int64_t)gridDim.x * blockDim.x)
     fr[A.slot_of[Q[i]]] = 0;
@@ -1518,8 +1530,8 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   st_ctrl(&c->ppairs, 0ull);
   st_ctrl(&c->mf, 0ull);
   st_ctrl(&c->live, 0ull);
-  st_ctrl(&c->segtail[0], 0ull);
-  st_ctrl(&c->segtail[1], 0ull);
+  st_ctrl(&c->segtail[0].v, 0ull);
+  st_ctrl(&c->segtail[1].v, 0ull);
   st_ctrl(&c->segtail_late, 0ull);
   reset_ms_cursors(A.cursor);
   if (cur_pm) st_ctrl(&c->noin_left, 0);  // this round's pull visited every vertex
@@ -1531,8 +1543,8 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   // listed only if this round did not pull, and a push part after a pull
   // rebuilds the list from an empty tail.
   const bool listed_next = cur_pm == 0;
-  st_ctrl(&c->qtail[d & 1], 0ull);
-  if (!listed_next) st_ctrl(&c->qtail[(d + 1) & 1], 0ull);
+  st_ctrl(&c->qtail[d & 1].v, 0ull);
+  if (!listed_next) st_ctrl(&c->qtail[(d + 1) & 1].v, 0ull);
   st_ctrl(&c->rounds, (long long)(d + 1));
   st_ctrl(&c->prev_nf, nf);
   st_ctrl(&c->gdir, g);
@@ -1691,8 +1703,8 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&L->dir[l], 0);
   }
   if (l == 0) {
-    st_ctrl(&c->qtail[0], 0ull);
-    st_ctrl(&c->qtail[1], 0ull);
+    st_ctrl(&c->qtail[0].v, 0ull);
+    st_ctrl(&c->qtail[1].v, 0ull);
     st_ctrl(&c->nf, 0ull);
     st_ctrl(&c->scanned, 0ull);
     st_ctrl(&c->rows, 0ull);
@@ -1703,8 +1715,8 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&c->ppairs, 0ull);
     st_ctrl(&c->mf, 0ull);
     st_ctrl(&c->live, 0ull);
-    st_ctrl(&c->segtail[0], 0ull);
-    st_ctrl(&c->segtail[1], 0ull);
+    st_ctrl(&c->segtail[0].v, 0ull);
+    st_ctrl(&c->segtail[1].v, 0ull);
     st_ctrl(&c->segtail_late, 0ull);
     st_ctrl(&c->pmask, 0ull);
     st_ctrl(&c->qmask, 0ull);
@@ -1813,7 +1825,7 @@ __global__ void eng_seed(MsArgs A, LaneOut *out, SeedSet S, MsParams P) {
     st_ctrl(&c->pmask, (unsigned long long)(ld_ctrl(&c->pmask) | pull));
     st_ctrl(&c->qmask, (unsigned long long)(ld_ctrl(&c->qmask) | (added & ~pull)));
     st_ctrl(&c->listed, 0);
-    st_ctrl(&c->qtail[d & 1], 0ull);
+    st_ctrl(&c->qtail[d & 1].v, 0ull);
     st_ctrl(&c->done, 0);
   }
 }
