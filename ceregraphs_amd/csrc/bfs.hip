@@ -22,8 +22,6 @@
 // four launches: bfs_level (the push or the pull), bfs_push_scan and
 // bfs_push_big (the push's vertices of out-degree >= big_degree, expanded
 // by the whole grid) and bfs_decide.
-#include <hip/hip_cooperative_groups.h>
-
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -62,6 +60,11 @@ constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the c
 #ifndef CG_CTRL_LINE
 #define CG_CTRL_LINE 128
 #endif
+constexpr int kBarGroups = 64;     // grid barrier: arrival counters, at most
+constexpr int kBarGroupSize = 32;  // grid barrier: blocks per counter (consecutive blockIdx)
+static_assert(kBarGroups <= 64, "bfs_init_source clears one barrier counter per thread of 64");
+constexpr unsigned kBarSpin = 1u << 22;  // grid barrier: polls before it gives up
+
 struct Ctrl {
   alignas(CG_CTRL_LINE) unsigned long long big_tail;  // push: count of whole-grid vertices
   alignas(CG_CTRL_LINE) unsigned long long nf;        // vertices discovered this round
@@ -81,6 +84,11 @@ struct Ctrl {
   long long rounds;
   unsigned long long t0;
   unsigned long long t_end;
+  // grid barrier of the persistent search (grid_barrier)
+  LineCtr bar_group[kBarGroups];
+  LineCtr bar_top;
+  LineCtr bar_gen;
+  alignas(CG_CTRL_LINE) int bar_abort;
 };
 
 // Per-search constants, passed by value to the kernels that need them.
@@ -173,7 +181,7 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
 // hipGetSymbolAddress, once per device, with that device current.
 constexpr int64_t kNoAllocMaxN = (int64_t)1 << 21;
 constexpr size_t kNoAllocBytes = ws1_size(kNoAllocMaxN, false);
-static_assert(kNoAllocBytes == 26371584, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
+static_assert(kNoAllocBytes == 26380032, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 constexpr int kMaxDevices = 64;
@@ -253,7 +261,13 @@ __global__ void __launch_bounds__(kBlock) bfs_init_fill(LevelArgs A) {
 }
 
 __global__ void bfs_init_source(LevelArgs A, int64_t s, BfsParams P, long long total_in) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (blockIdx.x != 0) return;
+  // the grid barrier's counters (one block of 64 threads: kBarGroups <= 64)
+  if (threadIdx.x < kBarGroups) st_ctrl(&A.ctrl->bar_group[threadIdx.x].v, 0ull);
+  if (threadIdx.x != 0) return;
+  st_ctrl(&A.ctrl->bar_top.v, 0ull);
+  st_ctrl(&A.ctrl->bar_gen.v, 0ull);
+  st_ctrl(&A.ctrl->bar_abort, 0);
   A.levels[s] = 0;
   A.parents[s] = (uint32_t)s;
   A.visited[s >> 5] |= 1u << (s & 31);
@@ -836,6 +850,65 @@ __global__ void __launch_bounds__(kBlock, 8) bfs_level(LevelArgs A, int d) {
   else pull_level(A, d);
 }
 
+// Grid-wide barrier of the persistent search (every block is resident: the
+// launch is cooperative).  A block arrives at the counter of its group
+// (kBarGroupSize consecutive blocks, each counter on its own line, so
+// arrivals serialise within a group only); the last arrival of a group
+// arrives at the top counter (none when the grid is one group), and the
+// last there advances the generation that every block polls.  The last
+// arrivals reset the counters, so barriers follow one another without
+// clearing.
+// Memory order: each arrival releases the block's writes (its threads'
+// stores reach L2 at the workgroup barrier; the release writes L2 back);
+// the last arrivals acquire what the others released before passing it on,
+// and every block acquires once it sees the new generation.  Waiting blocks
+// poll with relaxed loads and sleep between polls: an acquire per poll
+// would invalidate the caches under the blocks still working.
+// A barrier that sees no progress for kBarSpin polls, or finds that another
+// block gave up, sets bar_abort, done and an error status and returns
+// false; the caller leaves its level loop, so a fault ends the search with
+// CGBFS_EHIP instead of hanging it.  gen: the generation the caller last
+// passed (every thread keeps the same copy).
+__device__ bool grid_barrier(Ctrl *c, unsigned long long &gen) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nb = gridDim.x;
+    const unsigned g = blockIdx.x / kBarGroupSize;
+    const unsigned ng = (nb + kBarGroupSize - 1) / kBarGroupSize;
+    const unsigned rest = nb - g * kBarGroupSize;
+    const unsigned long long gsize = rest < (unsigned)kBarGroupSize ? rest : kBarGroupSize;
+    int ok = 1;
+    if (__hip_atomic_fetch_add(&c->bar_group[g].v, 1ull, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(&c->bar_group[g].v, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (ng == 1 || __hip_atomic_fetch_add(&c->bar_top.v, 1ull, __ATOMIC_RELEASE,
+                                            __HIP_MEMORY_SCOPE_AGENT) == ng - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __hip_atomic_store(&c->bar_top.v, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&c->bar_gen.v, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    unsigned spins = 0;
+    while (__hip_atomic_load(&c->bar_gen.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      if (ld_ctrl(&c->bar_abort) || ++spins > kBarSpin) {
+        st_ctrl(&c->bar_abort, 1);
+        st_ctrl(&c->status, (int)CGBFS_EHIP);
+        st_ctrl(&c->done, 1);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_ok = ok;
+  }
+  __syncthreads();
+  gen++;
+  return s_ok != 0;
+}
+
 // The whole level loop in one launch (cooperative: every block is resident,
 // so grid barriers are safe).  Each level is one round by the whole grid,
 // a grid barrier, the decide by one thread, and a second barrier, after
@@ -847,20 +920,20 @@ __global__ void __launch_bounds__(kBlock, 8) bfs_level(LevelArgs A, int d) {
 // barrier (taken by every block: the direction is uniform).  For small graphs, where the per-round launches of the level
 // kernels cost more than the rounds themselves.
 __global__ void __launch_bounds__(kBlock, 8) bfs_persistent(LevelArgs A, int64_t n, BfsParams P) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   Ctrl *c = A.ctrl;
+  unsigned long long gen = ld_ctrl(&c->bar_gen.v);  // no block passes a barrier before all read it
   for (long long d = 0; d <= P.max_levels; d++) {
     if (ld_ctrl(&c->done)) break;
     if (ld_ctrl(&c->dir) == 0) {  // uniform: every block read the same direction
       push_level_bitmap(A, (int)d);
-      grid.sync();
+      if (!grid_barrier(c, gen)) break;
       push_big_rows(A, (int)d);
     } else {
       pull_level(A, (int)d);
     }
-    grid.sync();
+    if (!grid_barrier(c, gen)) break;
     if (blockIdx.x == 0 && threadIdx.x == 0) decide_round(A, (int)d, n, P, (long long)ld_ctrl(&c->nf));
-    grid.sync();
+    if (!grid_barrier(c, gen)) break;
   }
 }
 
@@ -947,7 +1020,7 @@ using namespace cg;
 // hundreds of blocks) cost more than the launches they replace.
 constexpr int64_t kPersistentMaxN = (int64_t)1 << 14;
 
-// Caps *blocks at what the device holds resident at once for bfs_persistent.
+// Caps *blocks at what the device holds resident at once for bfs_persistent
 static int persistent_grid(int device, int *blocks) {
   static std::mutex mu;
   static int occ_of[kMaxDevices];
@@ -960,7 +1033,8 @@ static int persistent_grid(int device, int *blocks) {
     if (occ <= 0 || cus <= 0) return set_error(CGBFS_EHIP, "no resident blocks for the persistent search");
     occ_of[device] = occ * cus;
   }
-  *blocks = std::max(1, std::min(*blocks, occ_of[device]));
+  // ... and at what the grid barrier's counters cover
+  *blocks = std::max(1, std::min({*blocks, occ_of[device], kBarGroups * kBarGroupSize}));
   return CGBFS_OK;
 }
 
@@ -1053,6 +1127,7 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
       chunk = 4;
     }
   }
+  if (hc.status == CGBFS_EHIP) return set_error(hc.status, "a grid barrier of the persistent search gave up");
   if (hc.status) return set_error(hc.status, "BFS exceeded max_levels");
   const double hz = wall_clock_hz(g->device);
 

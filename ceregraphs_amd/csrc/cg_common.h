@@ -127,6 +127,13 @@ __device__ __forceinline__ void st_once(T *p, T v) {
   __builtin_nontemporal_store(v, gmem(p));
 }
 
+// A counter on a 128-byte line of its own.  Device-scope atomics on one line
+// are serialised, so counters that every block updates (round totals, queue
+// tails, barrier arrivals) each get a line.
+struct alignas(128) LineCtr {
+  unsigned long long v;
+};
+
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -81,11 +81,6 @@ int check_options(const cgbfs_options *o);
 // Round counters come in two parts: the push part (nf .. pairs) and the
 // pull part (pnf .. ppairs).  nf counts only vertices the pull did not
 // reach, so nf + pnf is the number of vertices with a new lane.
-// A counter on a 128-byte line of its own.
-struct alignas(128) LineCtr {
-  unsigned long long v;
-};
-
 // The round counters and tails each take a 128-byte line: every block of a
 // round kernel adds to them (or appends through them) once, and device-scope
 // atomics on one line are serialised, with each other and with the reads of

@@ -202,7 +202,7 @@ CGBFS_API int cgbfs_bfs(cgbfs_graph *g, int64_t source, int32_t *parents, int32_
  * (code run under HIP graph capture, tight serving loops, harnesses that
  * forbid allocation).  No counterpart in the reference.  Its workspace (the
  * frontier bitmaps, big-vertex list, control block and round history,
- * 26,371,584 bytes) is one file-scope device array per GPU, held from library load
+ * 26,380,032 bytes) is one file-scope device array per GPU, held from library load
  * until the process ends and sized for graphs of at most 2^21 vertices:
  * larger graphs get CGBFS_EINVAL (use cgbfs_bfs).  parents/levels: int32[n]
  * device arrays on the graph's device.  Results and statistics equal
