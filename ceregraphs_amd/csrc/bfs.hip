@@ -49,7 +49,7 @@ constexpr int kPullUnroll = CG_PULL_HEAD;  // pull: entries of each row the head
 constexpr int kPassSteps = CG_PULL_PASS_STEPS;  // pull: 64-entry steps per tail pass
 constexpr int kPass = kPassSteps * kWave;  // pull: tail entries per pass
 #ifndef CG_PULL_GRAB
-#define CG_PULL_GRAB 16
+#define CG_PULL_GRAB 8
 #endif
 constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the cursors (default)
 
@@ -973,7 +973,7 @@ double wall_clock_hz(int device) {
   return (double)khz * 1000.0;
 }
 
-void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o) {
+void fill_options(const cgbfs_options *opt, int64_t n, bool multi_source, cgbfs_options *o) {
   memset(o, 0, sizeof(*o));
   if (opt) *o = *opt;
   if (o->alpha <= 0) o->alpha = 14.0;
@@ -981,11 +981,17 @@ void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o) {
   if (o->tau <= 0) o->tau = (int64_t)(0.15 * (double)n + 0.5);
   if (o->max_levels <= 0) o->max_levels = n + 1;
   // Blocks per CU of the grid-stride kernels (and the cap of the pull's
-  // resident grid).  Measured on R-MAT ef16 graphs, single-root and 64-root
-  // searches alike: 2 is best up to 2^18 vertices, 4 from 2^19 to 2^22, 8
-  // from 2^23 (short rounds on small graphs pay for every extra block;
-  // large graphs need the full grid).
-  if (o->grid_mult <= 0) o->grid_mult = n < (1 << 19) ? 2 : n < (1 << 23) ? 4 : 8;
+  // resident grid): short rounds on small graphs pay for every extra block
+  // (each one's end-of-round probes and counter updates), large graphs need
+  // the full grid.  Measured on R-MAT ef16 graphs: 64-root passes are best
+  // at 2 up to 2^18 vertices, 4 from 2^19 to 2^22, 8 from 2^23; single-root
+  // searches (per search, 2 / 4 / 8 blocks per CU) s18 0.24 / 0.30 / 0.41
+  // ms, s20 0.30 / 0.34 / 0.45, s21 0.36 / 0.39 / 0.50, s22 0.50 / 0.48 /
+  // 0.57, s23 0.81 / 0.67 / 0.73, s24 1.45 / 1.07 / 1.04.
+  if (o->grid_mult <= 0) {
+    if (multi_source) o->grid_mult = n < (1 << 19) ? 2 : n < (1 << 23) ? 4 : 8;
+    else o->grid_mult = n <= (1 << 21) ? 2 : n <= (1 << 23) ? 4 : 8;
+  }
   if (o->seg_grab <= 0) o->seg_grab = 32;
   if (o->push_wave_degree <= 0) o->push_wave_degree = (int32_t)kWaveDegree;
   if (o->push_big_degree <= 0)
@@ -1014,11 +1020,12 @@ using namespace cg;
 
 // Graphs up to this many vertices run their levels in one persistent launch
 // by default (CGBFS_LOOP_AUTO).  Measured on R-MAT ef16 searches, device time
-// per search, per-level kernels / persistent: s10 0.19 / 0.10 ms, s12 0.22 /
-// 0.13, s14 0.26 / 0.22, s16 0.31 / 0.48, s18 0.37 / 1.03, s21 0.60 / 2.32:
-// beyond 2^14 vertices the grid barriers (two or three per level, over
-// hundreds of blocks) cost more than the launches they replace.
-constexpr int64_t kPersistentMaxN = (int64_t)1 << 14;
+// per search, per-level kernels / persistent: s10 (balanced 4x4) 0.15 /
+// 0.10 ms, s12 0.18 / 0.12, s14 0.19 / 0.15, s15 0.20 / 0.17, s16 0.21 /
+// 0.22, s21 0.41 / 0.87: beyond 2^15 vertices the grid barriers (two or
+// three per level, over hundreds of blocks) cost more than the four
+// launches per level they replace.
+constexpr int64_t kPersistentMaxN = (int64_t)1 << 15;
 
 // Caps *blocks at what the device holds resident at once for bfs_persistent
 static int persistent_grid(int device, int *blocks) {
@@ -1184,7 +1191,7 @@ static int bfs_args(cgbfs_graph *h, int64_t source, const void *parents, const v
   if (!h) return set_error(CGBFS_EINVAL, "graph is null");
   if (source < 0 || source >= h->g.n) return set_error(CGBFS_EINVAL, "source out of range");
   if (!parents || !levels) return set_error(CGBFS_EINVAL, "parents/levels must not be null");
-  fill_options(opt, h->g.n, o);
+  fill_options(opt, h->g.n, false, o);
   const int rc = check_options(o);
   if (rc) return rc;
   if (o->direction == CGBFS_DIR_BEAMER_LANES) o->direction = CGBFS_DIR_BEAMER;  // one lane

@@ -75,7 +75,7 @@ constexpr int kMsPullWaves = CG_MS_PULL_WAVES;
 using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
 int grid_blocks(int device, int mult);
-void fill_options(const cgbfs_options *opt, int64_t n, cgbfs_options *o);
+void fill_options(const cgbfs_options *opt, int64_t n, bool multi_source, cgbfs_options *o);
 int check_options(const cgbfs_options *o);
 
 // Round counters come in two parts: the push part (nf .. pairs) and the
@@ -1855,7 +1855,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
                   const cgbfs_options *opt, cgbfs_stats *stats, int64_t *m_lanes) {
   DeviceGraph *g = &h->g;
   cgbfs_options o;
-  fill_options(opt, g->n, &o);
+  fill_options(opt, g->n, true, &o);
   int rc0 = check_options(&o);
   if (rc0) return rc0;
   if (o.max_levels > 32767) o.max_levels = 32767;  // int16 levels
@@ -2250,7 +2250,7 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   if (!h || !out) return set_error(CGBFS_EINVAL, "graph and out must not be null");
   DeviceGraph *g = &h->g;
   cgbfs_options o;
-  fill_options(opt, g->n, &o);
+  fill_options(opt, g->n, true, &o);
   const int rc0 = check_options(&o);
   if (rc0) return rc0;
   // lanes run at different depths: the union-count rule becomes the per-lane one

@@ -74,10 +74,12 @@ typedef struct {
   double beta;        /* pull->push when n_f < n / beta               (24)   */
   int64_t tau;        /* CGBFS_DIR_TAU threshold on nf                (0.15n)*/
   int32_t pull_grab;  /* 64-vertex chunks a wavefront takes per grab in a
-                         dense pull round, 1..1024 (16 single-root,
+                         dense pull round, 1..1024 (8 single-root,
                          1 multi-source)                                    */
-  int32_t grid_mult;  /* blocks per CU for grid-stride kernels, 1..64 (2 below
-                         2^19 vertices, 4 below 2^23, else 8)              */
+  int32_t grid_mult;  /* blocks per CU for grid-stride kernels, 1..64
+                         (single root: 2 up to 2^21 vertices, 4 up to 2^23,
+                         else 8, multi-source: 2 below 2^19, 4 below 2^23,
+                         else 8)                                            */
   int64_t max_levels; /* abort with CGBFS_ETIMEOUT beyond this        (n+1)  */
   int32_t capture;    /* multi-source: capture the round kernels once into a
                          HIP graph and replay it, deciding every level on the
@@ -94,7 +96,7 @@ typedef struct {
 } cgbfs_options;
 
 /* How a single-root search runs its levels (cgbfs_options.loop). */
-#define CGBFS_LOOP_AUTO 0       /* persistent up to 2^14 vertices, else kernels */
+#define CGBFS_LOOP_AUTO 0       /* persistent up to 2^15 vertices, else kernels */
 #define CGBFS_LOOP_KERNELS 1    /* a fixed launch sequence per level, each kernel
                                    gated on the device's decision               */
 #define CGBFS_LOOP_PERSISTENT 2 /* one cooperative launch runs every level, grid
