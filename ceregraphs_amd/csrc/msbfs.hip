@@ -71,6 +71,14 @@ constexpr int32_t kGrabDense = CG_MS_DENSE_GRAB;  // chunks per grab of a dense 
 #define CG_MS_PULL_WAVES 7
 #endif
 constexpr int kMsPullWaves = CG_MS_PULL_WAVES;
+#ifndef CG_MS_PUSH_FILTER
+#define CG_MS_PUSH_FILTER 1
+#endif
+#ifndef CG_MS_PUSH_FILTER2
+#define CG_MS_PUSH_FILTER2 1
+#endif
+constexpr bool kPushFilter = CG_MS_PUSH_FILTER;    // push phase 0: read the word before atomicOr
+constexpr bool kPushFilter2 = CG_MS_PUSH_FILTER2;  // push phase 1: read the parent before atomicMin
 // First-hit offsets within a wavefront-tier row (< kSeg) fit in a byte.
 using TileOff = std::conditional_t<(kSeg <= 256), uint8_t, uint16_t>;
 
@@ -1338,9 +1346,14 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
       if (e < end) {
         w = A.out_idx[e];
         const int32_t sw = A.out_slot[e];
+        // Edges into one vertex meet at its word and its parent slots, and
+        // device-scope atomics on one address serialise: a hub's in-edges
+        // would queue there.  A plain read first skips the atomics that
+        // cannot change anything.  Within a pass bits are only set and
+        // parents only lowered, so a stale read at worst keeps an atomic.
         if (phase == 0) {
           uint64_t nv = f & ~A.vis[w];
-          if (nv) {
+          if (nv && (!kPushFilter || (nx[sw] & nv) != nv)) {
             uint64_t old = atomicOr((unsigned long long *)&nx[sw], (unsigned long long)nv);
             first = (old & qm) == 0;
           }
@@ -1349,8 +1362,9 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
           while (nv) {
             int l = __ffsll((unsigned long long)nv) - 1;
             nv &= nv - 1;
-            __hip_atomic_fetch_min(gmem((uint32_t *)&so.par[l][w]), (uint32_t)u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t *slot = (uint32_t *)&so.par[l][w];
+            if (!kPushFilter2 || (uint32_t)u < *gmem(slot))
+              __hip_atomic_fetch_min(gmem(slot), (uint32_t)u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
         }
       }
