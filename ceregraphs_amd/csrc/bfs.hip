@@ -60,6 +60,10 @@ constexpr int32_t kPullGrab = CG_PULL_GRAB;  // pull: chunks per grab from the c
 #ifndef CG_CTRL_LINE
 #define CG_CTRL_LINE 128
 #endif
+#ifndef CG_PARENT_FILTER
+#define CG_PARENT_FILTER 1
+#endif
+constexpr bool kParentFilter = CG_PARENT_FILTER;  // push: read a parent before its atomicMin
 constexpr int kBarGroups = 64;     // grid barrier: arrival counters, at most
 constexpr int kBarGroupSize = 32;  // grid barrier: blocks per counter (consecutive blockIdx)
 static_assert(kBarGroups <= 64, "bfs_init_source clears one barrier counter per thread of 64");
@@ -325,7 +329,9 @@ __device__ __forceinline__ void push_visit_bm(const LevelArgs &A, int d, uint32_
     atomicMin(&A.parents[v], u);
   } else {
     const int lv = __hip_atomic_load(&A.levels[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (lv == d + 1 || lv == -1) atomicMin(&A.parents[v], u);
+    // a parent already at or below u needs no atomic (parents only fall
+    // within a round, so a stale read at worst keeps one)
+    if ((lv == d + 1 || lv == -1) && (!kParentFilter || u < A.parents[v])) atomicMin(&A.parents[v], u);
   }
 }
 
