@@ -25,7 +25,9 @@ DIR_BEAMER_LANES = 4
 LOOP_AUTO = 0
 LOOP_KERNELS = 1
 LOOP_PERSISTENT = 2
-LOOPS = {"auto": LOOP_AUTO, "kernels": LOOP_KERNELS, "persistent": LOOP_PERSISTENT}
+LOOP_WORKGROUP = 3
+LOOPS = {"auto": LOOP_AUTO, "kernels": LOOP_KERNELS, "persistent": LOOP_PERSISTENT,
+         "workgroup": LOOP_WORKGROUP}
 DIRECTIONS = {"beamer": DIR_BEAMER, "topdown": DIR_TOPDOWN, "bottomup": DIR_BOTTOMUP,
               "tau": DIR_TAU, "beamer_lanes": DIR_BEAMER_LANES}
 

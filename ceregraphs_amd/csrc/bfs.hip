@@ -688,6 +688,23 @@ __device__ void pull_level(const LevelArgs &A, int d) {
 // The end of round d (one thread): records the round, resets the round
 // counters and chooses the next round's direction, or ends the search.
 // nf is the number of vertices the round reached.
+// The next round's direction (0 push, 1 pull) after a round in direction cur
+// that found nf vertices of out-degree sum mf, with mu in-edges of unvisited
+// vertices left and prev_nf found by the round before.
+__device__ __forceinline__ int next_direction(const BfsParams &P, int cur, long long nf,
+                                              long long mf, long long mu, long long prev_nf,
+                                              int64_t n) {
+  switch (P.mode) {
+    case CGBFS_DIR_TOPDOWN: return 0;
+    case CGBFS_DIR_BOTTOMUP: return 1;
+    case CGBFS_DIR_TAU:
+      return (cur == 0 && nf > P.tau) ? 1 : cur;  // one-way, bool_pe.csl:1078-1088
+    default:  // Beamer et al. alpha/beta
+      if (cur == 0) return (double)mf > (double)mu / P.alpha ? 1 : 0;
+      return ((double)nf < (double)n / P.beta && nf < prev_nf) ? 0 : 1;
+  }
+}
+
 __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsParams &P,
                              long long nf) {
   Ctrl *c = A.ctrl;
@@ -716,20 +733,7 @@ __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsPara
     st_ctrl(&c->done, 1);
     return;
   }
-  int dir = cur;
-  switch (P.mode) {
-    case CGBFS_DIR_TOPDOWN: dir = 0; break;
-    case CGBFS_DIR_BOTTOMUP: dir = 1; break;
-    case CGBFS_DIR_TAU:
-      if (dir == 0 && nf > P.tau) dir = 1;  // one-way, bool_pe.csl:1078-1088
-      break;
-    default:  // Beamer et al. alpha/beta
-      if (dir == 0) {
-        if ((double)mf > (double)mu / P.alpha) dir = 1;
-      } else {
-        if ((double)nf < (double)n / P.beta && nf < ld_ctrl(&c->prev_nf)) dir = 0;
-      }
-  }
+  const int dir = next_direction(P, cur, nf, mf, mu, ld_ctrl(&c->prev_nf), n);
   st_ctrl(&c->prev_nf, nf);
   // The next pull visits every chunk; once fewer vertices with in-edges are
   // unreached than there are chunks, most chunks have nothing to do.
@@ -748,6 +752,220 @@ __global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
   Ctrl *c = A.ctrl;
   if (ld_ctrl(&c->done)) return;
   decide_round(A, d, n, P, (long long)ld_ctrl(&c->nf));
+}
+
+// ---------------------------------------------------------------------------
+// Whole search in one workgroup (graphs of at most kTinyMaxN vertices)
+// ---------------------------------------------------------------------------
+// On graphs this small a level is a few hundred row entries, and any
+// grid-wide step -- a kernel boundary or a grid barrier -- costs more than
+// the level itself.  One workgroup of kTinyBlock threads runs every level:
+// the visited and frontier bitmaps, the parents (the atomicMin target) and
+// the levels stay in LDS, a level ends at a workgroup barrier, thread 0
+// decides with bfs_decide's rules (next_direction), and the outputs are
+// written once at the end.  A push lays each 64-vertex chunk's frontier rows
+// end to end (push_rows_packed); a pull scans each chunk's open rows in
+// packed 64-entry passes, the first hit in a row's stretch being its
+// lowest-id frontier in-neighbour.
+constexpr int kTinyBlock = 1024;
+constexpr int kTinyWaves = kTinyBlock / kWave;
+constexpr int64_t kTinyMaxN = 4096;
+constexpr int kTinyWords = (int)(kTinyMaxN / 32);
+
+template <class T>
+__device__ __forceinline__ T lds_load(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ void lds_store(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__global__ void __launch_bounds__(kTinyBlock) bfs_tiny(LevelArgs A, int64_t s, BfsParams P,
+                                                       long long total_in) {
+  __shared__ uint32_t vis[kTinyWords], fa[kTinyWords], fz[kTinyWords];
+  __shared__ uint32_t par[kTinyMaxN];
+  __shared__ int32_t lev[kTinyMaxN];
+  __shared__ int32_t rmap[kTinyWaves][kWave];
+  __shared__ unsigned long long cnt[5];  // found, mf, in_sum, scanned, rows
+  __shared__ int s_dir, s_done;
+  Ctrl *c = A.ctrl;
+  const int64_t n = A.n;
+  const int tid = threadIdx.x, lane = lane_id();
+  int32_t *rm = rmap[tid / kWave];
+  const int nwords = (int)((n + 31) / 32);
+  const int nchunks = (int)((n + 63) / 64);
+  const bool sym = A.symmetric != 0;
+  for (int w = tid; w < kTinyWords; w += kTinyBlock) {
+    vis[w] = 0;
+    fa[w] = 0;
+    fz[w] = 0;
+  }
+  for (int v = tid; v < n; v += kTinyBlock) {
+    par[v] = kParentNone;
+    lev[v] = -1;
+  }
+  __syncthreads();
+  long long mu = 0, prev_nf = 1;  // thread 0's decide state
+  if (tid == 0) {
+    vis[s >> 5] |= 1u << (s & 31);
+    fa[s >> 5] |= 1u << (s & 31);
+    par[s] = (uint32_t)s;
+    lev[s] = 0;
+    mu = total_in - (long long)(A.in_ptr[s + 1] - A.in_ptr[s]);
+    s_dir = P.mode == CGBFS_DIR_BOTTOMUP ? 1 : 0;  // round 0 pushes (bool_pe.csl:714-719)
+    s_done = 0;
+    st_ctrl(&c->status, 0);
+    st_ctrl(&c->done, 0);
+    st_ctrl(&c->rounds, 0ll);
+  }
+  uint32_t *cur = fa, *nxt = fz;
+  __syncthreads();
+  for (long long d = 0; d <= P.max_levels; d++) {
+    if (tid < 5) cnt[tid] = 0;
+    for (int w = tid; w < nwords; w += kTinyBlock) nxt[w] = 0;
+    __syncthreads();
+    const int dir = s_dir;
+    uint32_t found = 0, mf = 0, in_sum = 0, scanned = 0, rows = 0;
+    for (int ch = tid / kWave; ch < nchunks; ch += kTinyWaves) {
+      const int64_t v = (int64_t)ch * kWave + lane;
+      if (dir == 0) {
+        const bool on = v < n && ((cur[v >> 5] >> (v & 31)) & 1u);
+        if (!__any(on)) continue;
+        const int64_t beg = on ? A.out_ptr[v] : 0, end = on ? A.out_ptr[v + 1] : 0;
+        if (on) {
+          rows++;
+          scanned += (uint32_t)(end - beg);
+        }
+        push_rows_packed(A, rm, (uint32_t)v, beg, on ? (int32_t)(end - beg) : 0,
+                         [&](uint32_t u, int32_t w) {
+          const uint32_t b = 1u << (w & 31);
+          if (!(lds_load(&vis[w >> 5]) & b)) {
+            const uint32_t old = atomicOr(&vis[w >> 5], b);
+            if (!(old & b)) {
+              lds_store(&lev[w], (int32_t)(d + 1));
+              atomicOr(&nxt[w >> 5], b);
+              found++;
+              mf += (uint32_t)(A.out_ptr[w + 1] - A.out_ptr[w]);
+              in_sum += (uint32_t)(A.in_ptr[w + 1] - A.in_ptr[w]);
+            }
+            atomicMin(&par[w], u);
+          } else {
+            // claimed this round (level d + 1, or not yet written): offer u
+            const int32_t lv = lds_load(&lev[w]);
+            if (lv == d + 1 || lv == -1) atomicMin(&par[w], u);
+          }
+        });
+      } else {
+        // the chunk's visited words are this wavefront's alone in a pull
+        const bool todo = v < n && !((vis[v >> 5] >> (v & 31)) & 1u);
+        if (!__any(todo)) continue;
+        const int64_t beg = todo ? A.in_ptr[v] : 0, end = todo ? A.in_ptr[v + 1] : 0;
+        if (todo) rows++;
+        int64_t pos = beg, rem = end - beg;
+        bool hit = false;
+        uint32_t pu = 0;
+        while (__any(rem > 0)) {
+          const int32_t len = (int32_t)(rem < kWave ? rem : kWave);
+          const int32_t incl = wave_scan_add(len);
+          const int32_t vstart = incl - len;
+          const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+          rm[lane] = 0;
+          wave_sync();
+          if (len > 0 && vstart < kWave) rm[vstart] = lane + 1;
+          wave_sync();
+          const int r1 = wave_scan_max(rm[lane]);  // row + 1 of each position (0: none)
+          const bool ok = lane < total;
+          const int r = ok ? r1 - 1 : 0;
+          const int32_t vs = __shfl(vstart, r);
+          const int64_t ps = __shfl(pos, r);
+          const int32_t u = A.in_idx[ok ? ps + (lane - vs) : 0];
+          const uint64_t H = ballot64(ok && ((cur[u >> 5] >> (u & 31)) & 1u));
+          const bool mine = len > 0 && vstart < kWave;
+          const int32_t vstop = vstart + len < kWave ? vstart + len : kWave;
+          int32_t first = -1;
+          if (mine) {
+            const uint64_t inside = (vstop == kWave ? ~0ull : ((1ull << vstop) - 1ull)) &
+                                    ~((1ull << vstart) - 1ull);
+            const uint64_t h = H & inside;
+            if (h) first = __ffsll((unsigned long long)h) - 1;
+          }
+          const int32_t hid = __shfl(u, first >= 0 ? first : lane);
+          if (mine) {
+            if (first >= 0) {
+              hit = true;
+              pu = (uint32_t)hid;
+              scanned += (uint32_t)(first - vstart + 1);
+              rem = 0;
+            } else {
+              scanned += (uint32_t)(vstop - vstart);
+              pos += vstop - vstart;
+              rem -= vstop - vstart;
+            }
+          }
+        }
+        if (hit) {
+          par[v] = pu;
+          lev[v] = (int32_t)(d + 1);
+          found++;
+          mf += (uint32_t)(sym ? end - beg : A.out_ptr[v + 1] - A.out_ptr[v]);
+          in_sum += (uint32_t)(end - beg);
+        }
+        const uint64_t hm = ballot64(hit);
+        if (lane == 0 && hm) {
+          const int w0 = ch * 2;
+          const uint32_t lo = (uint32_t)hm, hi = (uint32_t)(hm >> 32);
+          if (lo) { nxt[w0] = lo; vis[w0] |= lo; }
+          if (hi) { nxt[w0 + 1] = hi; vis[w0 + 1] |= hi; }
+        }
+      }
+    }
+    // the round's counts: wavefront sums (32 bits: a tiny graph's round
+    // reads fewer than 2^31 entries), one LDS atomic per wavefront each
+    const uint32_t t[5] = {found, mf, in_sum, scanned, rows};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const int32_t tot = __builtin_amdgcn_readlane(wave_scan_add((int32_t)t[k]), kWave - 1);
+      if (lane == 0 && tot) atomicAdd(&cnt[k], (unsigned long long)(uint32_t)tot);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const long long nf = (long long)cnt[0];
+      mu -= (long long)cnt[2];
+      const unsigned long long now = wall_clock64();
+      if (d < kHistCap) {
+        A.hist->nf[d] = nf;
+        A.hist->scanned[d] = (long long)cnt[3];
+        A.hist->rows[d] = (long long)cnt[4];
+        A.hist->dir[d] = dir;
+        A.hist->t[d] = now;
+      }
+      st_ctrl(&c->rounds, (long long)(d + 1));
+      st_ctrl(&c->t_end, now);
+      int done = 0;
+      if (nf == 0) {
+        done = 1;
+      } else {
+        s_dir = next_direction(P, dir, nf, (long long)cnt[1], mu, prev_nf, n);
+        prev_nf = nf;
+        if (d + 1 >= P.max_levels) {
+          st_ctrl(&c->status, (int)CGBFS_ETIMEOUT);
+          done = 1;
+        }
+      }
+      s_done = done;
+    }
+    __syncthreads();
+    if (s_done) break;
+    uint32_t *t2 = cur;
+    cur = nxt;
+    nxt = t2;
+  }
+  for (int v = tid; v < n; v += kTinyBlock) {
+    A.levels[v] = lev[v];
+    A.parents[v] = par[v];
+  }
+  if (tid == 0) st_ctrl(&c->done, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1015,7 +1233,7 @@ int check_options(const cgbfs_options *o) {
   if (o->push_big_degree < o->push_wave_degree || o->push_big_degree > (1 << 30))
     return set_error(CGBFS_EINVAL, "push_big_degree must be in [push_wave_degree, 2^30]");
   if (!(o->alpha > 0) || !(o->beta > 0)) return set_error(CGBFS_EINVAL, "alpha and beta must be positive");
-  if (o->loop < CGBFS_LOOP_AUTO || o->loop > CGBFS_LOOP_PERSISTENT)
+  if (o->loop < CGBFS_LOOP_AUTO || o->loop > CGBFS_LOOP_WORKGROUP)
     return set_error(CGBFS_EINVAL, "unknown loop mode");
   return CGBFS_OK;
 }
@@ -1088,17 +1306,27 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
   P.tau = (long long)o.tau;
   P.max_levels = (long long)o.max_levels;
   P.n_in = (long long)g->n_in;
+  const bool tiny = o.loop == CGBFS_LOOP_WORKGROUP ||
+                    (o.loop == CGBFS_LOOP_AUTO && g->n <= kTinyMaxN);
+  const bool persistent = o.loop == CGBFS_LOOP_PERSISTENT ||
+                          (o.loop == CGBFS_LOOP_AUTO && g->n <= kPersistentMaxN && !tiny);
   bfs_start<<<1, 64, 0, st>>>(A.ctrl);
-  bfs_init_fill<<<blocks, kBlock, 0, st>>>(A);
-  bfs_init_source<<<1, 64, 0, st>>>(A, source, P, (long long)g->total_out_edges);
+  if (!tiny) {
+    bfs_init_fill<<<blocks, kBlock, 0, st>>>(A);
+    bfs_init_source<<<1, 64, 0, st>>>(A, source, P, (long long)g->total_out_edges);
+  }
   CG_HIP(hipGetLastError());
 
   Ctrl hc;
   hipEvent_t rev[2 * 64];  // per-round events of the first 64 rounds
   int nrev = 0;
-  const bool persistent = o.loop == CGBFS_LOOP_PERSISTENT ||
-                          (o.loop == CGBFS_LOOP_AUTO && g->n <= kPersistentMaxN);
-  if (persistent) {
+  if (tiny) {
+    // One workgroup runs the whole search and writes every output itself.
+    bfs_tiny<<<1, kTinyBlock, 0, st>>>(A, source, P, (long long)g->total_out_edges);
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    CG_HIP(hipStreamSynchronize(st));
+  } else if (persistent) {
     // One cooperative launch: as many blocks as give every wavefront a
     // chunk, at most the usual grid and at most what is resident at once.
     const int64_t nch = A.nwords / 2;
@@ -1201,6 +1429,8 @@ static int bfs_args(cgbfs_graph *h, int64_t source, const void *parents, const v
   const int rc = check_options(o);
   if (rc) return rc;
   if (o->direction == CGBFS_DIR_BEAMER_LANES) o->direction = CGBFS_DIR_BEAMER;  // one lane
+  if (o->loop == CGBFS_LOOP_WORKGROUP && h->g.n > kTinyMaxN)
+    return set_error(CGBFS_EINVAL, "the workgroup loop takes graphs of at most 4096 vertices");
   return CGBFS_OK;
 }
 

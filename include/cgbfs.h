@@ -96,11 +96,15 @@ typedef struct {
 } cgbfs_options;
 
 /* How a single-root search runs its levels (cgbfs_options.loop). */
-#define CGBFS_LOOP_AUTO 0       /* persistent up to 2^15 vertices, else kernels */
+#define CGBFS_LOOP_AUTO 0       /* workgroup up to 4096 vertices, persistent up to
+                                   2^15, else kernels                           */
 #define CGBFS_LOOP_KERNELS 1    /* a fixed launch sequence per level, each kernel
                                    gated on the device's decision               */
 #define CGBFS_LOOP_PERSISTENT 2 /* one cooperative launch runs every level, grid
                                    barriers between a level and its decision    */
+#define CGBFS_LOOP_WORKGROUP 3  /* one workgroup runs the whole search, its state
+                                   in LDS (graphs of at most 4096 vertices,
+                                   larger ones get CGBFS_EINVAL)                */
 
 /* Per-search statistics.  The caller owns the history arrays (capacity
  * max_hist entries each; any of them may be NULL).  Per-round meaning

@@ -431,3 +431,35 @@ def test_level_loops_equal_oracle(loop):
     with pytest.raises(_lib.CgbfsError) as e:
       dg.bfs(0, loop=loop, max_levels=10)
     assert e.value.code == _lib.CGBFS_ETIMEOUT
+
+
+def test_workgroup_loop_equal_oracle():
+  """The whole search in one workgroup (graphs of at most 4096 vertices,
+  the default up to that size) gives the oracle's levels, lowest-id
+  parents, per-round counts and m on undirected and directed graphs under
+  every direction policy, on a 1000-level path with a 399-way fan-out, and
+  with max_levels hit; a larger graph is refused."""
+  from ceregraphs_amd import _lib
+  for scale, ef, seed, directed in [(10, 16, 0, False), (12, 8, 3, False), (12, 16, 12, True),
+                                    (11, 32, 7, False)]:
+    abcd = (0.45, 0.25, 0.15, 0.15) if directed else gen_rmat.GRAPH500_ABCD
+    n, ip, ii, op, oi = gen_rmat.rmat_csr(scale, ef, seed, *abcd[:3], directed=directed)
+    g = O.Graph(n, ip, ii, op, oi, not directed)
+    deg = np.diff(op)
+    with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
+      for s in (int(np.argmax(deg)), int(np.nonzero(deg)[0][-1]), 0, n - 1):
+        for d in DIRECTIONS:
+          for loop in ("workgroup", "auto"):
+            _check_single(dg, g, s, d, loop=loop)
+  edges = [(i, i + 1) for i in range(1000)] + [(500, j) for j in range(1001, 1400)]
+  g = _edges_graph(edges, 1500, directed=False)
+  with _device(g) as dg:
+    for d in DIRECTIONS:
+      _check_single(dg, g, 0, d, loop="workgroup")
+    with pytest.raises(_lib.CgbfsError) as e:
+      dg.bfs(0, loop="workgroup", max_levels=10)
+    assert e.value.code == _lib.CGBFS_ETIMEOUT
+  with DeviceGraph.from_rmat(13, 8, 2) as dg:
+    with pytest.raises(_lib.CgbfsError) as e:
+      dg.bfs(0, loop="workgroup")
+    assert e.value.code == _lib.CGBFS_EINVAL
