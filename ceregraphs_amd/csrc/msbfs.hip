@@ -118,6 +118,9 @@ struct MsCtrl {
   int status;
   int grab;                    // chunks per grab of that round's pull; < 0: a static split
   int noin_left;               // outputs of vertices without in-edges not yet written
+  // Entries of the previous round's new-vertex list (q[rounds & 1]) whose
+  // push pairs wait in the side table for their outputs (0: none).
+  long long pend_n;
 };
 
 // Every round kernel reads its round from the control block (the previous
@@ -130,6 +133,7 @@ struct RoundState {
   bool listed, use_fb;
   int grab;
   bool noin;  // this round's pull also writes the outputs of vertices without in-edges
+  int64_t pend_n;  // previous round's push pairs waiting for their outputs (list length)
 };
 
 // Per-lane direction state.  cnt/out/in are this round's tallies of the
@@ -226,6 +230,20 @@ struct MsArgs {
   const int32_t *out_slot;
   uint32_t *cand;    // [pin.nbig][64] per-lane first-hit candidates
   uint64_t *bgot;    // [pin.nbig] lanes found this round
+  // Deferred push outputs.  A push round writes the parents of its new
+  // pairs into a compact side table instead of the output rows: per new
+  // vertex, a 64-bit lane mask and one candidate per lane (an even number of
+  // 32-bit words, so the mask is 8-byte aligned), taken from one of 64
+  // allocation counters (palloc, one 128-byte line each), each owning a
+  // 64th of the table.  pend[v] is the vertex's entry, -1 for none.  The
+  // next round writes the outputs with its pull's chunk stores (or with
+  // ms_push_out when it does not pull), so the scattered partial lines of
+  // a push's pairs are not written separately.  A vertex whose entry does
+  // not fit takes the direct path: +inf parents in the output rows.
+  int32_t *pend;     // [n]
+  uint32_t *side;    // [side_cap]
+  unsigned long long *palloc;  // [64 * 16]
+  int64_t side_cap;
   int64_t n;
   uint64_t act;      // mask of lanes in use
   int nroots;
@@ -269,7 +287,22 @@ struct MsWorkspace {
   int32_t *segq_late;
   uint64_t *segneed_late;
   unsigned long long *cursor;
+  int32_t *pend;
+  uint32_t *side;
+  unsigned long long *palloc;
+  int64_t side_cap;
 };
+
+// Words of the deferred-output side table: room for four new pairs per
+// vertex (a push round reaching every vertex in every lane falls back to
+// the direct path for what does not fit), between 2^17 and 2^27 words.
+static int64_t side_words(int64_t n) {
+  int64_t w = 4 * n;
+  if (w < (1ll << 17)) w = 1ll << 17;
+  if (w > (1ll << 27)) w = 1ll << 27;
+  return w & ~(int64_t)127;  // 64 regions of an even number of words
+}
+constexpr int kPallocStride = 16;  // u64 per allocation counter: one 128-byte line
 
 // ---------------------------------------------------------------------------
 // Per-graph segment lists (built on the host from the offset arrays once).
@@ -469,7 +502,8 @@ static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspac
       al(sizeof(uint32_t) * 64 * (nbig + 1)), al(sizeof(uint64_t) * (nbig + 1)),
       al(sizeof(int32_t) * (md->pin.nseg + 1)), al(sizeof(int32_t) * (md->pout.nseg + 1)),
       al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(int32_t) * (md->pin.nseg + 1)),
-      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(unsigned long long) * 3 * kParts * kCursorStride)};
+      al(sizeof(uint64_t) * (md->pin.nseg + 1)), al(sizeof(unsigned long long) * 3 * kParts * kCursorStride),
+      qb, al(sizeof(uint32_t) * side_words(n)), al(sizeof(unsigned long long) * 64 * kPallocStride)};
   size_t total = 0;
   for (size_t x : sizes) total += x;
   if (!base) return total;
@@ -495,6 +529,10 @@ static size_t ws_layout(int64_t n, const MsGraphData *md, char *base, MsWorkspac
   w->segq_late = (int32_t *)take();
   w->segneed_late = (uint64_t *)take();
   w->cursor = (unsigned long long *)take();
+  w->pend = (int32_t *)take();
+  w->side = (uint32_t *)take();
+  w->palloc = (unsigned long long *)take();
+  w->side_cap = side_words(n);
   return total;
 }
 
@@ -738,6 +776,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_init_fill(MsArgs A) {
   }
   for (int64_t i = tid; i < A.pin.nbig * 64; i += stride) A.cand[i] = 0xFFFFFFFFu;
   for (int64_t i = tid; i < A.pin.nbig; i += stride) A.bgot[i] = 0;
+  for (int64_t v = tid; v < A.n; v += stride) A.pend[v] = -1;
+  if (tid < 64) A.palloc[tid * kPallocStride] = 0;
 }
 
 __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
@@ -749,6 +789,7 @@ __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
   r.use_fb = ld_ctrl(&c->use_fb) != 0;
   r.grab = ld_ctrl(&c->grab);
   r.noin = ld_ctrl(&c->noin_left) != 0;
+  r.pend_n = (int64_t)ld_ctrl(&c->pend_n);
   return r;
 }
 
@@ -822,6 +863,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->status, 0);
     st_ctrl(&c->grab, P.dense_grab);
     st_ctrl(&c->noin_left, 1);
+    st_ctrl(&c->pend_n, 0ll);
     reset_ms_cursors(A.cursor);
   }
   bool live = false;
@@ -878,11 +920,16 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
 // the frontier bits are current (sparse frontiers only: on a dense one nearly
 // every bit is set and the extra load only adds latency) come from the
 // control block; tally_on keeps the per-lane tallies (per-lane policies).
-template <bool tally_on>
+// pend_on: the instantiation for a round after a push part that deferred
+// outputs (it writes them with its chunks); the other one runs every other
+// pull round, without the registers that costs.  Both are launched where
+// the host does not know the round (graph replay, the serving engine); the
+// one that does not match returns at once.
+template <bool tally_on, bool pend_on>
 __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
-  if (!R.pm) return;
+  if (!R.pm || (R.pend_n != 0) != pend_on) return;
   const int d = R.d;
   const uint64_t pm = R.pm;
   const bool use_fb = R.use_fb;
@@ -940,6 +987,15 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
     const uint64_t vw = A.vis[vv];
     const uint64_t need = (inr && !big) ? pm & ~vw : 0ull;
     const int64_t rb = rb0;
+    // Pairs the previous round's push left in the side table (lanes pl,
+    // entry pbase): written with this chunk's stores below.
+    int32_t pbase = -1;
+    uint64_t pl = 0;
+    if constexpr (pend_on) {
+      pbase = inr ? A.pend[vv] : -1;
+      const uint64_t m = *reinterpret_cast<const uint64_t *>(&A.side[pbase >= 0 ? pbase : 0]);
+      pl = pbase >= 0 ? m : 0ull;
+    }
     // A big vertex's row was pulled by segments (ms_pull_segs, before this
     // kernel): its new lanes and their parents wait in bgot / cand, indexed
     // by its rank among the big vertices (they ascend by vertex id).
@@ -1048,7 +1104,7 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
     // big vertices with new lanes (rare after the round that reaches the hubs)
     const bool bgot_any = __any(bg && got);
     // Flush: per lane bit, 64 consecutive vertices' parents and levels.
-    uint64_t bits = wave_or64(got | nmiss);
+    uint64_t bits = wave_or64(got | nmiss | pl);
     // First hits are the lowest ids with the bit, mostly near the row start.
     constexpr int kHead = 4;
     int32_t head[kHead];
@@ -1064,29 +1120,36 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
         lb[k] = bits ? __ffsll((unsigned long long)bits) - 1 : -1;
         bits &= bits - 1;
       }
-      bool hb[kFlush], nb[kFlush];
+      bool hb[kFlush], nb[kFlush], pb[kFlush];
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
-        hb[k] = lb[k] >= 0 && ((got >> lb[k]) & 1ull);
-        nb[k] = lb[k] >= 0 && ((nmiss >> lb[k]) & 1ull);
-        const int off = pt[lb[k] >= 0 ? lb[k] : 0][lane];
+        const int lk = lb[k] >= 0 ? lb[k] : 0;
+        hb[k] = lb[k] >= 0 && ((got >> lk) & 1ull);
+        nb[k] = lb[k] >= 0 && ((nmiss >> lk) & 1ull);
+        pb[k] = pend_on && lb[k] >= 0 && ((pl >> lk) & 1ull);
+        const int off = pt[lk][lane];
         // no branch: loads go out together; a hit among the row's first
-        // kHead entries takes its id from the row head loaded once per chunk
-        const int32_t far = A.in_idx[hb[k] && !bg && off >= kHead ? rb + off : 0];
-        pv[k] = off == 0 ? head[0] : off == 1 ? head[1] : off == 2 ? head[2] : off == 3 ? head[3] : far;
+        // kHead entries takes its id from the row head loaded once per chunk,
+        // a deferred push pair from its side entry
+        const int32_t *src =
+            pb[k] ? reinterpret_cast<const int32_t *>(A.side) + pbase + 2 + __popcll(pl & ((1ull << lk) - 1ull))
+                  : A.in_idx + (hb[k] && !bg && off >= kHead ? rb + off : 0);
+        const int32_t far = *src;
+        pv[k] = pb[k] || off >= kHead ? far : off == 0 ? head[0] : off == 1 ? head[1] : off == 2 ? head[2] : head[3];
         if (bgot_any) {  // a big vertex's parent is its candidate
-          const uint32_t cv = A.cand[bg && hb[k] ? (int64_t)bid * 64 + lb[k] : 0];
-          if (bg) pv[k] = (int32_t)cv;
+          const uint32_t cv = A.cand[bg && hb[k] ? (int64_t)bid * 64 + lk : 0];
+          if (bg && hb[k]) pv[k] = (int32_t)cv;
         }
       }
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
-        if (hb[k] || nb[k]) {
-          st_once(&so.par[lb[k]][v], hb[k] ? pv[k] : (int32_t)-1);
-          st_once(&so.lev[lb[k]][v], hb[k] ? so.level[lb[k]] : (int16_t)-1);
+        if (hb[k] || nb[k] || pb[k]) {
+          st_once(&so.par[lb[k]][v], hb[k] || pb[k] ? pv[k] : (int32_t)-1);
+          st_once(&so.lev[lb[k]][v], hb[k] ? so.level[lb[k]] : pb[k] ? (int16_t)(so.level[lb[k]] - 1) : (int16_t)-1);
         }
       }
     }
+    if (pend_on && pbase >= 0) A.pend[v] = -1;
     if (bg && got) {  // candidates back to +inf for the next pull
       uint4 *crow = reinterpret_cast<uint4 *>(&A.cand[(int64_t)bid * 64]);
 #pragma unroll
@@ -1367,13 +1430,22 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
             first = (old & qm) == 0;
           }
         } else {
-          uint64_t nv = f & nx[sw];
-          while (nv) {
-            int l = __ffsll((unsigned long long)nv) - 1;
-            nv &= nv - 1;
-            uint32_t *slot = (uint32_t *)&so.par[l][w];
-            if (!kPushFilter2 || (uint32_t)u < *gmem(slot))
-              __hip_atomic_fetch_min(gmem(slot), (uint32_t)u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          // the candidates of w's new lanes: its side entry (one per lane
+          // of nw, in lane order) or, without one, its output rows; the
+          // entry is loaded beside the word, not after it
+          const int32_t b = A.pend[w];
+          const uint64_t nxw = nx[sw];
+          uint64_t nv = f & nxw;
+          if (nv) {
+            const uint64_t nw = nxw & qm;
+            while (nv) {
+              int l = __ffsll((unsigned long long)nv) - 1;
+              nv &= nv - 1;
+              uint32_t *slot = b >= 0 ? &A.side[b + 2 + __popcll(nw & ((1ull << l) - 1ull))]
+                                      : (uint32_t *)&so.par[l][w];
+              if (!kPushFilter2 || (uint32_t)u < *gmem(slot))
+                __hip_atomic_fetch_min(gmem(slot), (uint32_t)u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
           }
         }
       }
@@ -1387,9 +1459,11 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
   if (phase == 0 && threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
 
-// Between the push passes: mark the new qm pairs (parent slot = +inf for the
-// atomicMin of pass 2, level = d+1), fold them into vis.  A vertex counts
-// towards nf and mf only if the pull reached no lane of it this round.
+// Between the push passes: give each new vertex its side-table entry (lane
+// mask, +inf candidates for the atomicMin of pass 2) or, where the table is
+// full, mark its new qm pairs in the output rows directly (parent +inf,
+// level d+1); fold the pairs into vis.  A vertex counts towards nf and mf
+// only if the pull reached no lane of it this round.
 __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
@@ -1409,22 +1483,53 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long nf = 0, pairs = 0, mf = 0;
   uint64_t live = 0;
-  // wavefront-uniform trip count (the tally is a wavefront operation)
-  for (int64_t i0 = gtid - lane_id(); i0 < nq; i0 += stride) {
-    const int64_t i = i0 + lane_id();
+  const int lane = lane_id();
+  // this wavefront's allocation counter and its region of the side table
+  const int64_t wave = gtid / kWave;
+  unsigned long long *pa = &A.palloc[(wave & 63) * kPallocStride];
+  const int64_t region = A.side_cap / 64;
+  const int64_t rbase = (wave & 63) * region;
+  // wavefront-uniform trip count (the tally and the allocation are
+  // wavefront operations)
+  for (int64_t i0 = gtid - lane; i0 < nq; i0 += stride) {
+    const int64_t i = i0 + lane;
     uint64_t nw = 0;
     int64_t od = 0, id = 0;
+    int32_t v = 0;
+    uint64_t all = 0;
     if (i < nq) {
-      const int32_t v = QN[i];
-      const int32_t sv = A.slot_of[v];
-      const uint64_t all = nx[sv];
+      v = QN[i];
+      all = nx[A.slot_of[v]];
       nw = all & qm;
-      uint64_t lanes = nw;
-      while (lanes) {
-        int l = __ffsll((unsigned long long)lanes) - 1;
-        lanes &= lanes - 1;
-        *gmem(&so.par[l][v]) = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
-        *gmem(&so.lev[l][v]) = so.level[l];
+    }
+    // side entry: the mask's two words and one per new lane, rounded up to
+    // an even count; one atomic per wavefront
+    const int32_t words = nw ? (2 + (int32_t)__popcll(nw) + 1) & ~1 : 0;
+    const int32_t incl = wave_scan_add(words);
+    const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    int64_t b = -1;
+    if (total) {
+      unsigned long long off = 0;
+      if (lane == 0) off = atomicAdd(pa, (unsigned long long)total);
+      off = (unsigned long long)shfl64((uint64_t)off, 0) + (unsigned long long)(incl - words);
+      if (words && (int64_t)off + words <= region) b = rbase + (int64_t)off;
+    }
+    if (i < nq) {
+      if (b >= 0) {
+        // the entry is 8-byte aligned (even offsets in even regions)
+        uint2 *e = reinterpret_cast<uint2 *>(&A.side[b]);
+        e[0] = make_uint2((uint32_t)nw, (uint32_t)(nw >> 32));
+        const int np2 = ((int)__popcll(nw) + 1) >> 1;
+        for (int k = 1; k <= np2; k++) e[k] = make_uint2(~0u, ~0u);  // +inf candidates
+        A.pend[v] = (int32_t)b;
+      } else {
+        uint64_t lanes = nw;
+        while (lanes) {
+          int l = __ffsll((unsigned long long)lanes) - 1;
+          lanes &= lanes - 1;
+          *gmem(&so.par[l][v]) = -1;  // 0xFFFFFFFF: +inf for the unsigned atomicMin
+          *gmem(&so.lev[l][v]) = so.level[l];
+        }
       }
       A.vis[v] |= nw;
       od = A.out_ptr[v + 1] - A.out_ptr[v];
@@ -1446,6 +1551,32 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
   if (threadIdx.x == 0 && s) atomicAdd(&c->pairs, s);
   s = ms_block_sum(mf, red);
   if (threadIdx.x == 0 && s) atomicAdd(&c->mf, s);
+}
+
+// The outputs of the pairs the previous round's push deferred to the side
+// table, in a round that does not pull (a pull writes them in its chunk
+// flush): one thread per vertex of that round's list, level d for each lane
+// of the entry's mask, the entry's candidate as the parent.
+__global__ void __launch_bounds__(kMsBlock) ms_push_out(MsArgs A) {
+  const RoundState R = round_state(A.ctrl);
+  if (!R.pend_n || R.pm) return;
+  __shared__ LaneOutLds so;
+  stage_out(A.out, R.d, so);
+  const int32_t *L = A.q[R.d & 1];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < R.pend_n; i += stride) {
+    const int32_t v = L[i];
+    const int32_t b = A.pend[v];
+    if (b < 0) continue;
+    uint64_t m = *reinterpret_cast<const uint64_t *>(&A.side[b]);
+    for (int k = 2; m; k++) {
+      const int l = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      *gmem(&so.par[l][v]) = (int32_t)A.side[b + k];
+      *gmem(&so.lev[l][v]) = (int16_t)(so.level[l] - 1);
+    }
+    A.pend[v] = -1;
+  }
 }
 
 // Clears the level-d frontier words (listed in Q[d]) so that buffer is zero
@@ -1522,6 +1653,10 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
     pm = g ? live : 0ull;
     qm = g ? 0ull : live;
   }
+  // the side table fills again from the start in the next push round; this
+  // round's entries are read (by the next round's pull or ms_push_out)
+  // before that round's ms_prep allocates
+  st_ctrl(&A.palloc[l * kPallocStride], 0ull);
   if (l != 0) return;
   const unsigned long long cur_pm = ld_ctrl(&c->pmask), cur_qm = ld_ctrl(&c->qmask);
   const long long pscanned = (long long)ld_ctrl(&c->pscanned);
@@ -1561,6 +1696,8 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   // listed only if this round did not pull, and a push part after a pull
   // rebuilds the list from an empty tail.
   const bool listed_next = cur_pm == 0;
+  // the vertices this round's push part listed hold the pairs it deferred
+  st_ctrl(&c->pend_n, cur_qm ? (long long)ld_ctrl(&c->qtail[(d + 1) & 1].v) : 0ll);
   st_ctrl(&c->qtail[d & 1].v, 0ull);
   if (!listed_next) st_ctrl(&c->qtail[(d + 1) & 1].v, 0ull);
   st_ctrl(&c->rounds, (long long)(d + 1));
@@ -1747,6 +1884,7 @@ __global__ void eng_init(MsArgs A) {
     st_ctrl(&c->status, 0);
     st_ctrl(&c->grab, (int)kGrabDense);
     st_ctrl(&c->noin_left, 0);  // retiring lanes write their own
+    st_ctrl(&c->pend_n, 0ll);
     reset_ms_cursors(A.cursor);
   }
 }
@@ -1862,10 +2000,14 @@ using namespace cg;
 // once.  The pull hands out its chunks dynamically, so blocks that could only
 // start after others finish would add a tail and nothing else.
 static int pull_grid(int device, int blocks, bool tally) {
-  int occ = 0, cus = 0;
-  const hipError_t e = tally
-      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms_pull<true>, kMsBlock, 0)
-      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms_pull<false>, kMsBlock, 0);
+  int occ = 0, occ2 = 0, cus = 0;
+  hipError_t e = tally
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms_pull<true, false>, kMsBlock, 0)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms_pull<false, false>, kMsBlock, 0);
+  if (e == hipSuccess)
+    e = tally ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, ms_pull<true, true>, kMsBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, ms_pull<false, true>, kMsBlock, 0);
+  occ = std::min(occ, occ2);
   if (e != hipSuccess || occ <= 0 ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
       cus <= 0)
@@ -1943,6 +2085,10 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   A.segq_late = ws.segq_late;
   A.segneed_late = ws.segneed_late;
   A.cursor = ws.cursor;
+  A.pend = ws.pend;
+  A.side = ws.side;
+  A.palloc = ws.palloc;
+  A.side_cap = ws.side_cap;
   A.noin = md->noin;
   A.host_done = nullptr;
   A.n = g->n;
@@ -1987,6 +2133,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   ms_init_seeds<<<1, 64, 0, st>>>(A, dsv, dsl, nseeds, P);
   MsCtrl hc;
   std::memset(&hc, 0, sizeof(hc));
+  const bool replay = o.capture != 0;
   struct RoundEvents { hipEvent_t a, k, p, b; };
   std::vector<RoundEvents> rev;
   const int tally = lane_policy(P.mode) ? 1 : 0;
@@ -2005,8 +2152,15 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
       ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 0);
       ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 1);
     }
-    if (tally) ms_pull<true><<<pblocks, kMsBlock, 0, st>>>(A);
-    else ms_pull<false><<<pblocks, kMsBlock, 0, st>>>(A);
+    // both instantiations when the round is not known here (replay)
+    const bool np = hc.pend_n == 0 || replay, pp = hc.pend_n != 0 || replay;
+    if (tally) {
+      if (np) ms_pull<true, false><<<pblocks, kMsBlock, 0, st>>>(A);
+      if (pp) ms_pull<true, true><<<pblocks, kMsBlock, 0, st>>>(A);
+    } else {
+      if (np) ms_pull<false, false><<<pblocks, kMsBlock, 0, st>>>(A);
+      if (pp) ms_pull<false, true><<<pblocks, kMsBlock, 0, st>>>(A);
+    }
   };
   auto push_part = [&](bool compact) {
     if (compact) ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
@@ -2042,6 +2196,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
       e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
       if (e == hipSuccess) {
         for (int r = 0; r < kGraphRounds; r++) {
+          ms_push_out<<<blocks, kMsBlock, 0, st>>>(A);
           pull_part(true);
           pull_kernels();
           push_part(true);
@@ -2075,6 +2230,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
           (e = new_event(&re.p)) != hipSuccess || (e = new_event(&re.b)) != hipSuccess)
         break;
       hipEventRecord(re.a, st);
+      if (hc.pend_n && !hc.pmask) ms_push_out<<<blocks, kMsBlock, 0, st>>>(A);
       if (hc.pmask) pull_part(hc.use_fb != 0);
       hipEventRecord(re.k, st);
       if (hc.pmask) pull_kernels();
@@ -2251,14 +2407,20 @@ static void eng_round(cgbfs_engine *E) {
     int64_t k = (nb + kMsBlock - 1) / kMsBlock;
     return (unsigned)(k < blocks ? k : blocks);
   };
+  ms_push_out<<<blocks, kMsBlock, 0, st>>>(A);
   ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
   if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
   if (md->pin.nbig) {
     ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 0);
     ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 1);
   }
-  if (E->tally) ms_pull<true><<<E->pblocks, kMsBlock, 0, st>>>(A);
-  else ms_pull<false><<<E->pblocks, kMsBlock, 0, st>>>(A);
+  if (E->tally) {
+    ms_pull<true, false><<<E->pblocks, kMsBlock, 0, st>>>(A);
+    ms_pull<true, true><<<E->pblocks, kMsBlock, 0, st>>>(A);
+  } else {
+    ms_pull<false, false><<<E->pblocks, kMsBlock, 0, st>>>(A);
+    ms_pull<false, true><<<E->pblocks, kMsBlock, 0, st>>>(A);
+  }
   ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
   if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
   ms_push<<<blocks, kMsBlock, 0, st>>>(A, 0);
@@ -2324,6 +2486,10 @@ extern "C" int cgbfs_engine_create(cgbfs_graph *h, const cgbfs_options *opt, cgb
   A.segq_late = E->ws.segq_late;
   A.segneed_late = E->ws.segneed_late;
   A.cursor = E->ws.cursor;
+  A.pend = E->ws.pend;
+  A.side = E->ws.side;
+  A.palloc = E->ws.palloc;
+  A.side_cap = E->ws.side_cap;
   A.noin = E->md->noin;
   A.host_done = nullptr;
   A.n = g->n;
