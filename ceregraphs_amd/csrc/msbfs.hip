@@ -1245,10 +1245,10 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A, int late) {
 
 // Active list from the frontier words (slot order -> vertex ids), for a push
 // round that follows a pull round (pull rounds keep no list).
-__global__ void __launch_bounds__(kMsBlock) ms_compact(MsArgs A, const int32_t *vertex_of) {
+// Lists the round's frontier (its nonzero words, by vertex) for a push part
+// after a pull part, which lists nothing.  Every thread of the block calls it.
+__device__ void compact_part(const MsArgs &A, const RoundState &R, const int32_t *vertex_of) {
   MsCtrl *c = A.ctrl;
-  const RoundState R = round_state(c);
-  if (!R.qm || R.listed) return;
   const int d = R.d;
   const uint64_t *fr = A.fw[d & 1];
   __shared__ int32_t lq[kMsLdsQueue];
@@ -1268,10 +1268,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_compact(MsArgs A, const int32_t *
 // This round's active segments: those of the big vertices that still need
 // a pull lane (which = 0, in view) or that are on the push lanes' frontier
 // (which = 1, out view).  One wavefront-aggregated append per wavefront.
-__global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int which) {
-  const RoundState R = round_state(A.ctrl);
+__device__ void active_segs_part(const MsArgs &A, const RoundState &R, int which) {
   const uint64_t mask = which ? R.qm : R.pm;
-  if (!mask) return;
   const int d = R.d;
   const SegList &L = which ? A.pout : A.pin;
   const uint64_t *fr = A.fw[d & 1];
@@ -1325,9 +1323,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_active_segs(MsArgs A, int which) 
 // Frontier bits from the frontier words: a wavefront ballots 64 consecutive
 // slots and stores the two bitmap words (whole words: no atomics, and no
 // clearing between rounds).
-__global__ void __launch_bounds__(kMsBlock) ms_mark(MsArgs A) {
-  const RoundState R = round_state(A.ctrl);
-  if (!R.pm || !R.use_fb) return;
+__device__ void mark_part(const MsArgs &A, const RoundState &R) {
   const uint64_t *__restrict__ fr = A.fw[R.d & 1];
   uint32_t *__restrict__ fb = A.fb;
   const int64_t n = A.n;
@@ -1340,9 +1336,9 @@ __global__ void __launch_bounds__(kMsBlock) ms_mark(MsArgs A) {
   }
 }
 
-// Whether the level-d frontier words are cleared through their list
-// (ms_clear): only when the round's frontier is listed and the list is
-// short.  Otherwise ms_clear_all sweeps every word: a listed word costs a
+// Whether the level-d frontier words are cleared through their list: only
+// when the round's frontier is listed and the list is short.  Otherwise
+// ms_clear sweeps every word: a listed word costs a
 // gather and a scattered 8-byte store, a swept one a share of a streamed
 // line (a 3.3 M-vertex list at 2^24 vertices took 115 us, the sweep 25 us).
 __device__ __forceinline__ bool clear_by_list(const MsArgs &A, const RoundState &R) {
@@ -1350,15 +1346,6 @@ __device__ __forceinline__ bool clear_by_list(const MsArgs &A, const RoundState 
   return (int64_t)ld_ctrl(&A.ctrl->qtail[R.d & 1].v) * 32 <= A.n;
 }
 
-// Clears the whole level-d frontier-word buffer when ms_clear does not.
-__global__ void __launch_bounds__(kMsBlock) ms_clear_all(MsArgs A) {
-  const RoundState R = round_state(A.ctrl);
-  if (clear_by_list(A, R)) return;
-  uint64_t *w = A.fw[R.d & 1];
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < A.n;
-       k += (int64_t)gridDim.x * blockDim.x)
-    w[k] = 0;
-}
 
 // Push pass for the lanes in qm (phase 0: OR into nx and append the first
 // writer of qm bits; phase 1: offer predecessors to the new pairs with
@@ -1557,9 +1544,8 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
 // table, in a round that does not pull (a pull writes them in its chunk
 // flush): one thread per vertex of that round's list, level d for each lane
 // of the entry's mask, the entry's candidate as the parent.
-__global__ void __launch_bounds__(kMsBlock) ms_push_out(MsArgs A) {
-  const RoundState R = round_state(A.ctrl);
-  if (!R.pend_n || R.pm) return;
+// Every thread of the block calls it.
+__device__ void push_out_part(const MsArgs &A, const RoundState &R) {
   __shared__ LaneOutLds so;
   stage_out(A.out, R.d, so);
   const int32_t *L = A.q[R.d & 1];
@@ -1579,19 +1565,43 @@ __global__ void __launch_bounds__(kMsBlock) ms_push_out(MsArgs A) {
   }
 }
 
-// Clears the level-d frontier words (listed in Q[d]) so that buffer is zero
-// when it becomes the next-words buffer of round d+1.
+// Clears the level-d frontier words so that buffer is zero when it becomes
+// the next-words buffer of round d+1: through their list (Q[d]) when it is
+// short, else by sweeping every word.
 __global__ void __launch_bounds__(kMsBlock) ms_clear(MsArgs A) {
   MsCtrl *c = A.ctrl;
   const RoundState R = round_state(c);
-  if (!clear_by_list(A, R)) return;
   const int d = R.d;
   uint64_t *fr = A.fw[d & 1];
-  const int32_t *Q = A.q[d & 1];
-  const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1].v);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
-       i += (int64_t)gridDim.x * blockDim.x)
-    fr[A.slot_of[Q[i]]] = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (clear_by_list(A, R)) {
+    const int32_t *Q = A.q[d & 1];
+    const int64_t nq = (int64_t)ld_ctrl(&c->qtail[d & 1].v);
+    for (int64_t i = tid; i < nq; i += stride) fr[A.slot_of[Q[i]]] = 0;
+  } else {
+    for (int64_t k = tid; k < A.n; k += stride) fr[k] = 0;
+  }
+}
+
+// First kernel of a round: the outputs the previous round's push deferred
+// (when this round does not pull; a pull writes them with its chunks), the
+// frontier bits and the active segments of the pull part.  Each part gates
+// itself on the control block.
+__global__ void __launch_bounds__(kMsBlock) ms_begin(MsArgs A) {
+  const RoundState R = round_state(A.ctrl);
+  if (R.pend_n && !R.pm) push_out_part(A, R);
+  if (R.pm && R.use_fb) mark_part(A, R);
+  if (R.pm && A.pin.nbig) active_segs_part(A, R, 0);
+}
+
+// First kernel of a push part: the frontier list after a pull part, and
+// the active segments of the push lanes' big frontier vertices.
+__global__ void __launch_bounds__(kMsBlock) ms_push_begin(MsArgs A, const int32_t *vertex_of) {
+  const RoundState R = round_state(A.ctrl);
+  if (!R.qm) return;
+  if (!R.listed) compact_part(A, R, vertex_of);
+  if (A.pout.nbig) active_segs_part(A, R, 1);
 }
 
 // One wavefront; ends round d.  Lanes that reached nothing this round drop
@@ -2137,16 +2147,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   struct RoundEvents { hipEvent_t a, k, p, b; };
   std::vector<RoundEvents> rev;
   const int tally = lane_policy(P.mode) ? 1 : 0;
-  auto sblocks = [&](int64_t nb) {
-    int64_t k = (nb + kMsBlock - 1) / kMsBlock;
-    return (unsigned)(k < blocks ? k : blocks);
-  };
   // Every kernel of a round gates itself on the control block, so launching
   // a part that is off costs only an empty dispatch.
-  auto pull_part = [&](bool fb) {
-    if (fb) ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
-    if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
-  };
   auto pull_kernels = [&]() {
     if (md->pin.nbig) {
       ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 0);
@@ -2162,9 +2164,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
       if (pp) ms_pull<false, true><<<pblocks, kMsBlock, 0, st>>>(A);
     }
   };
-  auto push_part = [&](bool compact) {
-    if (compact) ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
-    if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
+  auto push_part = [&](bool begin) {
+    if (begin) ms_push_begin<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
     ms_push<<<blocks, kMsBlock, 0, st>>>(A, 0);
     ms_prep<<<blocks, kMsBlock, 0, st>>>(A, tally);
     ms_push<<<blocks, kMsBlock, 0, st>>>(A, 1);
@@ -2196,12 +2197,10 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
       e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
       if (e == hipSuccess) {
         for (int r = 0; r < kGraphRounds; r++) {
-          ms_push_out<<<blocks, kMsBlock, 0, st>>>(A);
-          pull_part(true);
+          ms_begin<<<blocks, kMsBlock, 0, st>>>(A);
           pull_kernels();
           push_part(true);
           ms_clear<<<blocks, kMsBlock, 0, st>>>(A);
-          ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A);
           ms_decide<<<1, 64, 0, st>>>(A, P);
         }
         hipError_t ec = hipStreamEndCapture(st, &graph);
@@ -2230,16 +2229,14 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
           (e = new_event(&re.p)) != hipSuccess || (e = new_event(&re.b)) != hipSuccess)
         break;
       hipEventRecord(re.a, st);
-      if (hc.pend_n && !hc.pmask) ms_push_out<<<blocks, kMsBlock, 0, st>>>(A);
-      if (hc.pmask) pull_part(hc.use_fb != 0);
+      if (hc.pmask || hc.pend_n) ms_begin<<<blocks, kMsBlock, 0, st>>>(A);
       hipEventRecord(re.k, st);
       if (hc.pmask) pull_kernels();
       hipEventRecord(re.p, st);
-      if (hc.qmask) push_part(!hc.listed);
+      if (hc.qmask) push_part(!hc.listed || md->pout.nbig);
       hipEventRecord(re.b, st);
       rev.push_back(re);
-      if (hc.listed || hc.qmask) ms_clear<<<blocks, kMsBlock, 0, st>>>(A);
-      ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A);
+      ms_clear<<<blocks, kMsBlock, 0, st>>>(A);
       ms_decide<<<1, 64, 0, st>>>(A, P);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
@@ -2403,13 +2400,7 @@ static void eng_round(cgbfs_engine *E) {
   hipStream_t st = E->st;
   const int blocks = E->blocks;
   MsGraphData *md = E->md;
-  auto sblocks = [&](int64_t nb) {
-    int64_t k = (nb + kMsBlock - 1) / kMsBlock;
-    return (unsigned)(k < blocks ? k : blocks);
-  };
-  ms_push_out<<<blocks, kMsBlock, 0, st>>>(A);
-  ms_mark<<<blocks, kMsBlock, 0, st>>>(A);
-  if (md->pin.nbig) ms_active_segs<<<sblocks(md->pin.nbig), kMsBlock, 0, st>>>(A, 0);
+  ms_begin<<<blocks, kMsBlock, 0, st>>>(A);
   if (md->pin.nbig) {
     ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 0);
     ms_pull_segs<<<blocks, kMsBlock, 0, st>>>(A, 1);
@@ -2421,13 +2412,11 @@ static void eng_round(cgbfs_engine *E) {
     ms_pull<false, false><<<E->pblocks, kMsBlock, 0, st>>>(A);
     ms_pull<false, true><<<E->pblocks, kMsBlock, 0, st>>>(A);
   }
-  ms_compact<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
-  if (md->pout.nbig) ms_active_segs<<<sblocks(md->pout.nbig), kMsBlock, 0, st>>>(A, 1);
+  ms_push_begin<<<blocks, kMsBlock, 0, st>>>(A, md->vertex_of);
   ms_push<<<blocks, kMsBlock, 0, st>>>(A, 0);
   ms_prep<<<blocks, kMsBlock, 0, st>>>(A, E->tally);
   ms_push<<<blocks, kMsBlock, 0, st>>>(A, 1);
   ms_clear<<<blocks, kMsBlock, 0, st>>>(A);
-  ms_clear_all<<<blocks, kMsBlock, 0, st>>>(A);
   ms_decide<<<1, 64, 0, st>>>(A, E->P);
 }
 

@@ -249,15 +249,16 @@ class DeviceGraph:
     self._last_rounds = int(sb.s.rounds_completed)
     return BFSResult(int(source), par, lev, sb.result())
 
-  def msbfs(self, roots, direction="beamer", device_out=None, **kw):
-    """Up to 64 searches in one pass; lane l is root roots[l]."""
+  def msbfs(self, roots, direction="beamer", device_out=None, count_m=True, **kw):
+    """Up to 64 searches in one pass; lane l is root roots[l].  count_m:
+    also count each lane's GTEPS m on the device (m_lanes; None without)."""
     roots = np.ascontiguousarray(roots, dtype=np.int64)
     k = len(roots)
     if not 1 <= k <= 64:
       raise ValueError("msbfs takes 1..64 roots")
     sb = _StatsBuf()
     opt = _options(direction, **kw)
-    m = np.zeros(k, np.int64)
+    m = np.zeros(k, np.int64) if count_m else None
     if device_out is not None:
       par, lev = device_out
       _check_tensor(par, k * self.n, "int32")
