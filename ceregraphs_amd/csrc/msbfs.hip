@@ -166,14 +166,17 @@ struct MsParams {
   int dense_grab;  // chunks per grab of a dense pull round
 };
 
+// Per-round statistics, one row per round, so a search's rounds are one
+// contiguous copy.
+struct MsHistRow {
+  long long nf, scanned, rows, pairs;
+  long long pnf, pscanned, prows, ppairs;
+  long long plane;  // lanes pulled
+  int dir;          // 0 push only, 1 pull only, 2 both
+  int pad;
+};
 struct MsHist {
-  long long nf[kMsHist];
-  long long scanned[kMsHist];
-  long long rows[kMsHist];
-  long long pairs[kMsHist];
-  long long pnf[kMsHist], pscanned[kMsHist], prows[kMsHist], ppairs[kMsHist];
-  long long plane[kMsHist];  // lanes pulled
-  int dir[kMsHist];          // 0 push only, 1 pull only, 2 both
+  MsHistRow r[kMsHist];
 };
 
 // Static segment list of one adjacency view (per graph).
@@ -267,8 +270,13 @@ struct MsGraphData {
   // arguments it was captured with; reused while they stay the same.
   hipGraphExec_t exec = nullptr;
   std::vector<char> exec_key;
-  void *seedbuf = nullptr;  // the call's seed vertices and lanes
-  int64_t seedcap = 0;
+  // The call's lane output table, seed vertices and seed lanes, in that
+  // order (the table's address stays fixed while the buffer does), and a
+  // pinned host copy they are staged in, so one asynchronous copy moves them.
+  void *seedbuf = nullptr;
+  void *seedstage = nullptr;
+  size_t seedcap = 0;
+  bool stage_busy = false;  // a copy from the staging buffer may be in flight
 };
 
 struct MsWorkspace {
@@ -475,6 +483,7 @@ static void free_msdata(void *q) {
   MsGraphData *m = (MsGraphData *)q;
   if (m->exec) hipGraphExecDestroy(m->exec);
   if (m->seedbuf) hipFree(m->seedbuf);
+  if (m->seedstage) hipHostFree(m->seedstage);
   if (m->mem) hipFree(m->mem);
   if (m->noin) hipFree(m->noin);
   if (m->out_slot && m->out_slot != m->in_slot) hipFree(m->out_slot);
@@ -720,17 +729,55 @@ static_assert(kSeg % kWave == 0, "a segment is a whole number of wavefront steps
 // loads of all steps are in flight together.  Each lane's first hit in the
 // segment (its original id) is offered to cand[] with atomicMin.  Returns
 // the lanes found (wavefront-uniform).
+// One step of a segment pull: entries [beg + 64 k, ...) below end, first
+// hits into cand (atomicMin), returns the lanes found.
+__device__ __forceinline__ uint64_t seg_step(const MsArgs &A, int64_t beg, int k, uint64_t h,
+                                             uint32_t *cand) {
+  const int lane = lane_id();
+  const uint64_t inc = wave_scan_or(h);
+  const uint64_t before = lane_before(inc);
+  uint64_t mine = h & ~before;
+  if (mine) {  // the id is read only for a first hit (h != 0: the entry is in range)
+    const uint32_t u = (uint32_t)A.in_idx[beg + k * kWave + lane];
+    while (mine) {
+      int l = __ffsll((unsigned long long)mine) - 1;
+      mine &= mine - 1;
+      atomicMin(&cand[l], u);
+    }
+  }
+  return shfl64(inc, kWave - 1);
+}
+
+// A segment needing at most this many lanes looks at its first step alone
+// first: in late rounds the few lanes a hub still needs are mostly found
+// among its first, lowest-id in-neighbours (hubs themselves).
+constexpr int kSegHeadLanes = 8;
+
 __device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t *fr, int64_t beg,
                                                 int64_t end, uint64_t need, uint32_t *cand,
                                                 bool use_fb, uint32_t &scanned) {
   const int lane = lane_id();
+  uint64_t got = 0;
+  int k0 = 0;
+  if (__popcll(need) <= kSegHeadLanes) {
+    const int64_t e = beg + lane;
+    bool ok = e < end;
+    const int32_t sl = A.in_slot[ok ? e : beg];
+    if (use_fb) ok = ok && ((A.fb[sl >> 5] >> (sl & 31)) & 1u);
+    const uint64_t w = fr[ok ? sl : 0];
+    if (lane == 0) scanned += (uint32_t)(end - beg < kWave ? end - beg : kWave);
+    const uint64_t h = ok ? w & need : 0ull;
+    if (__any(h != 0)) got = seg_step(A, beg, 0, h, cand);
+    if (got == need || end - beg <= kWave) return got;
+    k0 = 1;
+  }
   int32_t sl[kSegSteps];
   bool ok[kSegSteps];
   uint64_t w[kSegSteps];
 #pragma unroll
   for (int k = 0; k < kSegSteps; k++) {
     const int64_t e = beg + k * kWave + lane;
-    ok[k] = e < end;
+    ok[k] = k >= k0 && e < end;
     const int64_t ee = ok[k] ? e : beg;
     sl[k] = A.in_slot[ee];
   }
@@ -740,24 +787,12 @@ __device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t 
   }
 #pragma unroll
   for (int k = 0; k < kSegSteps; k++) w[k] = fr[ok[k] ? sl[k] : 0];
-  if (lane == 0) scanned += (uint32_t)(end - beg);
-  uint64_t got = 0;
+  if (lane == 0) scanned += (uint32_t)(end - beg - (int64_t)k0 * kWave);
 #pragma unroll
   for (int k = 0; k < kSegSteps; k++) {
     const uint64_t h = ok[k] ? w[k] & need & ~got : 0ull;
     if (!__any(h != 0)) continue;  // no frontier in-neighbour in this step
-    const uint64_t inc = wave_scan_or(h);
-    const uint64_t before = lane_before(inc);
-    uint64_t mine = h & ~before;
-    if (mine) {  // the id is read only for a first hit (h != 0: the entry is in range)
-      const uint32_t u = (uint32_t)A.in_idx[beg + k * kWave + lane];
-      while (mine) {
-        int l = __ffsll((unsigned long long)mine) - 1;
-        mine &= mine - 1;
-        atomicMin(&cand[l], u);
-      }
-    }
-    got |= shfl64(inc, kWave - 1);
+    got |= seg_step(A, beg, k, h, cand);
     if (got == need) break;
   }
   return got;
@@ -1005,12 +1040,32 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
       bid = A.pin.chunk_big[ch] + (int32_t)__popcll(bigm & ((1ull << lane) - 1ull));
     const int32_t len = need ? (int32_t)(re0 - rb0) : 0;  // < kSeg for non-big rows
     if (len) rows++;
-    // exclusive prefix sum of row lengths -> virtual starts
-    const int32_t incl = wave_scan_add(len);
-    const int32_t vstart = incl - len;
-    const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    // Head probe: the first entry of every row that needs lanes, all at
+    // once.  In a late round most rows end here: a row's lowest in-neighbours
+    // are hubs, reached early.  The rows still open are then laid end to end
+    // from their second entry.
     uint64_t got = 0;
-    const int32_t my_end = vstart + len;
+    {
+      const int32_t hs = A.in_slot[len ? rb : 0];
+      bool hok = len != 0;
+      if (use_fb) hok = hok && ((A.fb[hs >> 5] >> (hs & 31)) & 1u);
+      const uint64_t hw = fr[hok ? hs : 0];
+      got = hok ? hw & need : 0ull;
+      if (len) scanned++;
+      uint64_t m = got;
+      while (m) {
+        const int l = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        pt[l][lane] = 0;
+      }
+    }
+    const int32_t lenr = (len && (need & ~got)) ? len - 1 : 0;  // entries left to scan
+    const int64_t rbr = rb + 1;
+    // exclusive prefix sum of the rest of the rows -> virtual starts
+    const int32_t incl = wave_scan_add(lenr);
+    const int32_t vstart = incl - lenr;
+    const int32_t total = __builtin_amdgcn_readlane(incl, kWave - 1);
+    const int32_t my_end = vstart + lenr;
     // Row of the entries before the current window: the windows run in
     // order, so it is the row of the previous window's last lane.
     int rcarry = 0;
@@ -1032,7 +1087,7 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
       wave_sync();
       {
         const int32_t p = vstart - base0;
-        if (len && p >= 0 && p < kWave * kUnroll) rm[p / kWave][p % kWave] = lane;
+        if (lenr && p >= 0 && p < kWave * kUnroll) rm[p / kWave][p % kWave] = lane;
       }
       wave_sync();
 #pragma unroll
@@ -1046,7 +1101,7 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
         const int32_t ve = base0 + k * kWave + lane;
         const int r = rk[k];
         rsk[k] = __shfl(vstart, r);
-        const int64_t rbase = __shfl(rb, r);
+        const int64_t rbase = __shfl(rbr, r);
         nrk[k] = shfl64(need, r);
         const uint64_t gr = shfl64(got, r);
         ok[k] = ve < total && (nrk[k] & ~gr);
@@ -1080,7 +1135,7 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
         const uint64_t x = seg_scan_or(h, head, lane);
         const uint64_t prev = lane_before(x);
         uint64_t mine = h & ~(lane > head ? prev : 0ull);
-        const TileOff off = (TileOff)(base + lane - rs);
+        const TileOff off = (TileOff)(base + lane - rs + 1);  // from the row start
         while (mine) {
           int l = __ffsll((unsigned long long)mine) - 1;
           mine &= mine - 1;
@@ -1088,11 +1143,11 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
         }
         // each row owner collects its row's OR over this step
         int last = (my_end < base + kWave ? my_end : base + kWave) - 1 - base;
-        bool hit_step = len && vstart < base + kWave && my_end > base;
+        bool hit_step = lenr && vstart < base + kWave && my_end > base;
         uint64_t tot = shfl64(x, hit_step ? last : lane);
         if (hit_step) got |= tot;
       }
-      if (!__any(len && (need & ~got))) break;  // every row of the chunk satisfied
+      if (!__any(lenr && (need & ~got))) break;  // every row of the chunk satisfied
     }
     // Vertices without in-edges: no lane but a seed's own ever reaches them.
     // The first pull round writes their -1 outputs (lanes other than their
@@ -1671,17 +1726,17 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   const unsigned long long cur_pm = ld_ctrl(&c->pmask), cur_qm = ld_ctrl(&c->qmask);
   const long long pscanned = (long long)ld_ctrl(&c->pscanned);
   if (d < kMsHist) {
-    MsHist *H = A.hist;
-    H->nf[d] = nf;
-    H->pnf[d] = pnf;
-    H->pscanned[d] = pscanned;
-    H->prows[d] = (long long)ld_ctrl(&c->prows);
-    H->ppairs[d] = (long long)ld_ctrl(&c->ppairs);
-    H->scanned[d] = H->pscanned[d] + (long long)ld_ctrl(&c->scanned);
-    H->rows[d] = H->prows[d] + (long long)ld_ctrl(&c->rows);
-    H->pairs[d] = H->ppairs[d] + (long long)ld_ctrl(&c->pairs);
-    H->plane[d] = __popcll(cur_pm);
-    H->dir[d] = cur_pm ? (cur_qm ? 2 : 1) : 0;
+    MsHistRow *H = &A.hist->r[d];
+    H->nf = nf;
+    H->pnf = pnf;
+    H->pscanned = pscanned;
+    H->prows = (long long)ld_ctrl(&c->prows);
+    H->ppairs = (long long)ld_ctrl(&c->ppairs);
+    H->scanned = H->pscanned + (long long)ld_ctrl(&c->scanned);
+    H->rows = H->prows + (long long)ld_ctrl(&c->rows);
+    H->pairs = H->ppairs + (long long)ld_ctrl(&c->pairs);
+    H->plane = __popcll(cur_pm);
+    H->dir = cur_pm ? (cur_qm ? 2 : 1) : 0;
   }
   st_ctrl(&c->nf, 0ull);
   st_ctrl(&c->scanned, 0ull);
@@ -2040,15 +2095,22 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   int rc = ensure_wsm(g, &ws, &md);
   if (rc) return rc;
   hipStream_t st = g->stream;
-  if (md->seedcap < nseeds) {
+  // a call that failed after enqueueing its copy may not have waited for it
+  if (md->stage_busy) CG_HIP(hipStreamSynchronize(st));
+  md->stage_busy = false;
+  const size_t sv_at = al(sizeof(LaneOut)), sl_at = sv_at + al(sizeof(int64_t) * nseeds);
+  const size_t seed_bytes = sl_at + sizeof(int32_t) * nseeds;
+  if (md->seedcap < seed_bytes) {
     if (md->seedbuf) hipFree(md->seedbuf);
-    md->seedbuf = nullptr;
+    if (md->seedstage) hipHostFree(md->seedstage);
+    md->seedbuf = md->seedstage = nullptr;
     md->seedcap = 0;
-    CG_HIP(hipMalloc(&md->seedbuf, al(sizeof(int64_t) * nseeds) + sizeof(int32_t) * nseeds));
-    md->seedcap = nseeds;
+    CG_HIP(hipMalloc(&md->seedbuf, seed_bytes));
+    CG_HIP(hipHostMalloc(&md->seedstage, seed_bytes, hipHostMallocDefault));
+    md->seedcap = seed_bytes;
   }
-  int64_t *dsv = (int64_t *)md->seedbuf;
-  int32_t *dsl = (int32_t *)((char *)md->seedbuf + al(sizeof(int64_t) * nseeds));
+  int64_t *dsv = (int64_t *)((char *)md->seedbuf + sv_at);
+  int32_t *dsl = (int32_t *)((char *)md->seedbuf + sl_at);
 
   const size_t plane = (size_t)g->n * nroots;
   int32_t *dpar = parents;
@@ -2066,13 +2128,18 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   A.out_idx = g->out_idx;
   A.in_ptr = g->in_ptr;
   A.in_idx = g->in_idx;
-  A.out = ws.out;
-  LaneOut lo;
+  A.out = (LaneOut *)md->seedbuf;
+  // staged: the output table, then the seeds (the previous call has
+  // finished with the staging buffer: every call ends synchronised)
+  char *stage = (char *)md->seedstage;
+  LaneOut &lo = *(LaneOut *)stage;
   std::memset(&lo, 0, sizeof(lo));
   for (int l = 0; l < nroots; l++) {
     lo.par[l] = dpar + (size_t)l * g->n;
     lo.lev[l] = dlev + (size_t)l * g->n;
   }
+  std::memcpy(stage + sv_at, sv, sizeof(int64_t) * nseeds);
+  std::memcpy(stage + sl_at, sl, sizeof(int32_t) * nseeds);
   A.vis = ws.vis;
   A.fw[0] = ws.fw[0];
   A.fw[1] = ws.fw[1];
@@ -2120,9 +2187,8 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     if (tmp) hipFree(tmp);
   };
   hipEvent_t e0, e1;
-  hipError_t e = hipMemcpyAsync(dsv, sv, sizeof(int64_t) * nseeds, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(ws.out, &lo, sizeof(lo), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dsl, sl, sizeof(int32_t) * nseeds, hipMemcpyHostToDevice, st);
+  hipError_t e = hipMemcpyAsync(md->seedbuf, stage, seed_bytes, hipMemcpyHostToDevice, st);
+  md->stage_busy = true;
   if (e == hipSuccess) e = new_event(&e0);
   if (e == hipSuccess) e = new_event(&e1);
   if (e == hipSuccess) e = hipEventRecord(e0, st);
@@ -2251,6 +2317,7 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   }
   if (e == hipSuccess) e = hipEventRecord(e1, st);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) md->stage_busy = false;
   float ms = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
   if (e != hipSuccess || hc.status) {
@@ -2287,24 +2354,24 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
     int64_t nh = hc.rounds < kMsHist ? hc.rounds : kMsHist;
     if (stats->max_hist < nh) nh = stats->max_hist;
     if (nh > 0) {
-      std::vector<char> hb(sizeof(MsHist));
-      e = hipMemcpy(hb.data(), A.hist, sizeof(MsHist), hipMemcpyDeviceToHost);
+      std::vector<MsHistRow> hb((size_t)nh);
+      e = hipMemcpy(hb.data(), A.hist->r, sizeof(MsHistRow) * nh, hipMemcpyDeviceToHost);
       if (e != hipSuccess) {
         cleanup();
         return hip_fail(e, "msbfs stats", __FILE__, __LINE__);
       }
-      const MsHist *H = (const MsHist *)hb.data();
       for (int64_t r = 0; r < nh; r++) {
-        if (stats->nf_history) stats->nf_history[r] = H->nf[r];
-        if (stats->direction_history) stats->direction_history[r] = H->dir[r];
-        if (stats->edges_scanned) stats->edges_scanned[r] = H->scanned[r];
-        if (stats->rows_scanned) stats->rows_scanned[r] = H->rows[r];
-        if (stats->pairs_found) stats->pairs_found[r] = H->pairs[r];
-        if (stats->pull_lanes) stats->pull_lanes[r] = H->plane[r];
-        if (stats->pull_nf) stats->pull_nf[r] = H->pnf[r];
-        if (stats->pull_rows) stats->pull_rows[r] = H->prows[r];
-        if (stats->pull_edges) stats->pull_edges[r] = H->pscanned[r];
-        if (stats->pull_pairs) stats->pull_pairs[r] = H->ppairs[r];
+        const MsHistRow *H = &hb[(size_t)r];
+        if (stats->nf_history) stats->nf_history[r] = H->nf;
+        if (stats->direction_history) stats->direction_history[r] = H->dir;
+        if (stats->edges_scanned) stats->edges_scanned[r] = H->scanned;
+        if (stats->rows_scanned) stats->rows_scanned[r] = H->rows;
+        if (stats->pairs_found) stats->pairs_found[r] = H->pairs;
+        if (stats->pull_lanes) stats->pull_lanes[r] = H->plane;
+        if (stats->pull_nf) stats->pull_nf[r] = H->pnf;
+        if (stats->pull_rows) stats->pull_rows[r] = H->prows;
+        if (stats->pull_edges) stats->pull_edges[r] = H->pscanned;
+        if (stats->pull_pairs) stats->pull_pairs[r] = H->ppairs;
         if (r < (int64_t)rev.size()) {
           float rm = 0, pms = 0;
           hipEventElapsedTime(&rm, rev[r].a, rev[r].b);
