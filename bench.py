@@ -218,24 +218,30 @@ def parse_args():
   return p.parse_args()
 
 
-def per_root_leg(dg, roots, device_out, warm=2):
+def per_root_leg(dg, roots, device_out, warm=2, capture=True):
+  """One search per root.  capture: the per-level loop replays its captured
+  HIP graph (the timed leg); without it every level is bracketed by events
+  (the instrumented leg, whose level times give the roofline)."""
   import torch
   for s in roots[:warm]:
-    dg.bfs(int(s), device_out=device_out)
+    dg.bfs(int(s), device_out=device_out, capture=capture)
   torch.cuda.synchronize()
   res = []
   tw = time.perf_counter()
   for s in roots:
-    res.append(dg.bfs(int(s), device_out=device_out))
+    res.append(dg.bfs(int(s), device_out=device_out, capture=capture))
   torch.cuda.synchronize()
   return res, time.perf_counter() - tw
 
 
-def per_root_summary(n, res, wall):
+def per_root_summary(n, res, wall, inst=None):
+  """Search times and m of the timed leg; the roofline of the dominant
+  levels from the instrumented leg inst (the same roots, events per level),
+  whose outputs must equal the timed leg's."""
   tms = np.array([r.stats.total_ms for r in res])
   m = np.array([r.stats.m_edges for r in res], dtype=np.float64)
   hm = harmonic_mean(m / (tms / 1e3) / 1e9)
-  roof = per_root_roofline(n, res)
+  roof = per_root_roofline(n, inst if inst is not None else res)
   traffic, src = traffic_from_profile("bfs_pull_s24_traffic.json", "dominant_level_hbm_bytes")
   if n == 1 << 24:
     roof["traffic"], roof["traffic_source"] = traffic, src
@@ -292,7 +298,8 @@ def main():
       for _ in range(args.steps):
         r, w = per_root_leg(dg, roots, (p1, l1), warm=0)
         res, wall = res + r, wall + w
-      summary, m, tms = per_root_summary(n, res, wall)
+      inst, _ = per_root_leg(dg, roots, (p1, l1), warm=0, capture=False)
+      summary, m, tms = per_root_summary(n, res, wall, inst)
       value = summary["hmean_gteps"]
       ms_step = wall / len(res) * 1e3
       metric = "harmonic-mean GTEPS, RMAT s21 ef16, 64 roots, one search per launch"
@@ -302,7 +309,8 @@ def main():
       for _ in range(args.warmup):
         dg.bfs(s, device_out=(p1, l1))
       res, wall = per_root_leg(dg, [s] * args.steps, (p1, l1), warm=0)
-      summary, m, tms = per_root_summary(n, res, wall)
+      inst, _ = per_root_leg(dg, [s], (p1, l1), warm=0, capture=False)
+      summary, m, tms = per_root_summary(n, res, wall, inst)
       value = summary["hmean_gteps"]
       ms_step = wall / len(res) * 1e3
       metric = "GTEPS, RMAT s10 ef16 balanced 4x4, source 0"
@@ -403,7 +411,8 @@ def main():
     p1 = torch.empty(n, dtype=torch.int32, device="cuda")
     l1 = torch.empty(n, dtype=torch.int32, device="cuda")
     res, wall_roots = per_root_leg(dg, roots, (p1, l1))
-    summary, ms_root, tms = per_root_summary(n, res, wall_roots)
+    inst, _ = per_root_leg(dg, roots, (p1, l1), warm=0, capture=False)
+    summary, ms_root, tms = per_root_summary(n, res, wall_roots, inst)
     if not np.array_equal(ms_root, m):
       log("WARNING: per-root m differs from the multi-source lanes")
     extra.update({

@@ -401,10 +401,10 @@ def main(argv=None):
     p1 = torch.empty(n, dtype=torch.int32, device="cuda")
     l1 = torch.empty(n, dtype=torch.int32, device="cuda")
     dg.msbfs(roots[g], capture=True, device_out=(gp, gl))
-    dg.bfs(int(roots[g][0]), device_out=(p1, l1))
+    dg.bfs(int(roots[g][0]), device_out=(p1, l1), capture=True)
     t_graph += dg.msbfs(roots[g], capture=True, device_out=(gp, gl)).stats.total_ms
     for l in range(lanes):
-      t_root += dg.bfs(int(roots[g][l]), device_out=(p1, l1)).stats.total_ms
+      t_root += dg.bfs(int(roots[g][l]), device_out=(p1, l1), capture=True).stats.total_ms
 
   msum = float(m.sum())
   report = {"workload": "uneven-8-directed-rmat", "searches": int(nsearch),

@@ -103,12 +103,15 @@ struct BfsParams {
   long long n_in;  // vertices with at least one in-edge
 };
 
+// Per-round statistics, one row per round (a search's rounds are one copy).
+struct HistRow {
+  long long nf, scanned, rows;
+  unsigned long long t;  // device clock at the round's decide
+  int dir;
+  int pad;
+};
 struct Hist {
-  long long nf[kHistCap];
-  long long scanned[kHistCap];
-  long long rows[kHistCap];
-  unsigned long long t[kHistCap];
-  int dir[kHistCap];
+  HistRow r[kHistCap];
 };
 
 struct Workspace {
@@ -185,7 +188,7 @@ static int ensure_ws1(DeviceGraph *g, Workspace *w) {
 // hipGetSymbolAddress, once per device, with that device current.
 constexpr int64_t kNoAllocMaxN = (int64_t)1 << 21;
 constexpr size_t kNoAllocBytes = ws1_size(kNoAllocMaxN, false);
-static_assert(kNoAllocBytes == 26380032, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
+static_assert(kNoAllocBytes == 26396416, "the size stated for cgbfs_bfs_noalloc in cgbfs.h");
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 constexpr int kMaxDevices = 64;
@@ -417,9 +420,10 @@ __global__ void __launch_bounds__(1024) bfs_push_scan(LevelArgs A) {
 // numbered 0..big_edges-1 in list order and every thread takes flat indices
 // gtid, gtid + stride, ...; the owning vertex comes from a binary search
 // over the degree prefix staged in LDS (kBigLds entries per pass).
-__global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A, int d) {
+__global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A) {
   Ctrl *c = A.ctrl;
   if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
+  const int d = (int)ld_ctrl(&c->rounds);
   const int64_t nb = (int64_t)ld_ctrl(&c->big_tail);
   if (nb == 0) return;
   __shared__ unsigned long long red[kWavesPerBlock];
@@ -713,11 +717,12 @@ __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsPara
   const long long mu = ld_ctrl(&c->mu) - (long long)ld_ctrl(&c->in_sum);
   unsigned long long now = wall_clock64();
   if (d < kHistCap) {
-    A.hist->nf[d] = nf;
-    A.hist->scanned[d] = (long long)ld_ctrl(&c->scanned);
-    A.hist->rows[d] = (long long)ld_ctrl(&c->rows);
-    A.hist->dir[d] = cur;
-    A.hist->t[d] = now;
+    HistRow &H = A.hist->r[d];
+    H.nf = nf;
+    H.scanned = (long long)ld_ctrl(&c->scanned);
+    H.rows = (long long)ld_ctrl(&c->rows);
+    H.dir = cur;
+    H.t = now;
   }
   st_ctrl(&c->rounds, (long long)(d + 1));
   st_ctrl(&c->t_end, now);
@@ -747,11 +752,14 @@ __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsPara
   }
 }
 
-__global__ void bfs_decide(LevelArgs A, int d, int64_t n, BfsParams P) {
+// The round kernels read their round from the control block (the previous
+// decide advanced it), so one launch sequence serves every round and can be
+// captured once into a HIP graph.
+__global__ void bfs_decide(LevelArgs A, int64_t n, BfsParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Ctrl *c = A.ctrl;
   if (ld_ctrl(&c->done)) return;
-  decide_round(A, d, n, P, (long long)ld_ctrl(&c->nf));
+  decide_round(A, (int)ld_ctrl(&c->rounds), n, P, (long long)ld_ctrl(&c->nf));
 }
 
 // ---------------------------------------------------------------------------
@@ -934,11 +942,12 @@ __global__ void __launch_bounds__(kTinyBlock) bfs_tiny(LevelArgs A, int64_t s, B
       mu -= (long long)cnt[2];
       const unsigned long long now = wall_clock64();
       if (d < kHistCap) {
-        A.hist->nf[d] = nf;
-        A.hist->scanned[d] = (long long)cnt[3];
-        A.hist->rows[d] = (long long)cnt[4];
-        A.hist->dir[d] = dir;
-        A.hist->t[d] = now;
+        HistRow &H = A.hist->r[d];
+        H.nf = nf;
+        H.scanned = (long long)cnt[3];
+        H.rows = (long long)cnt[4];
+        H.dir = dir;
+        H.t = now;
       }
       st_ctrl(&c->rounds, (long long)(d + 1));
       st_ctrl(&c->t_end, now);
@@ -1067,9 +1076,10 @@ __device__ void push_big_rows(const LevelArgs &A, int d) {
 // out-degree >= big_degree left to bfs_push_big) or the pull, as the
 // previous decide chose.  One launch serves both, so a level costs one
 // dispatch less than a kernel per direction.
-__global__ void __launch_bounds__(kBlock, 8) bfs_level(LevelArgs A, int d) {
+__global__ void __launch_bounds__(kBlock, 8) bfs_level(LevelArgs A) {
   const Ctrl *c = A.ctrl;
   if (ld_ctrl(&c->done)) return;
+  const int d = (int)ld_ctrl(&c->rounds);
   if (ld_ctrl(&c->dir) == 0) push_level_bitmap(A, d);
   else pull_level(A, d);
 }
@@ -1273,6 +1283,17 @@ static int persistent_grid(int device, int *blocks) {
 // to device arrays par / lev (the caller's or the workspace's).  events: time
 // each round's traversal kernels with HIP events; without them a round's time
 // is the span between its decide and the one before (device clock).
+// One level of the per-level loop: the push or the pull, the push's big
+// vertices, the decide.  Each kernel gates itself on the control block.
+static void level_round(const LevelArgs &A, int blocks, int64_t n, const BfsParams &P,
+                        hipStream_t st) {
+  bfs_level<<<blocks, kBlock, 0, st>>>(A);
+  bfs_push_scan<<<1, 1024, 0, st>>>(A);
+  bfs_push_big<<<blocks, kBlock, 0, st>>>(A);
+  bfs_decide<<<1, 64, 0, st>>>(A, n, P);
+}
+constexpr int kReplayRounds = 12;  // rounds per captured graph
+
 static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t *lev,
                    uint32_t *par, const cgbfs_options &o, cgbfs_stats *stats, bool events) {
   LevelArgs A;
@@ -1338,6 +1359,43 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
     CG_HIP(hipLaunchCooperativeKernel((const void *)bfs_persistent, dim3(pblocks), dim3(kBlock), args, 0, st));
     CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     CG_HIP(hipStreamSynchronize(st));
+  } else if (o.capture && events) {
+    // Graph replay: kReplayRounds rounds of the level kernels are captured
+    // once (every kernel reads its round and direction from the control
+    // block), the instantiated graph is kept on the handle while the launch
+    // arguments match, and it is replayed until the done flag is up: one
+    // host poll per replay, no per-round events.
+    std::vector<char> key(sizeof(A) + sizeof(P) + sizeof(int) + sizeof(int64_t));
+    std::memcpy(key.data(), &A, sizeof(A));
+    std::memcpy(key.data() + sizeof(A), &P, sizeof(P));
+    std::memcpy(key.data() + sizeof(A) + sizeof(P), &blocks, sizeof(int));
+    std::memcpy(key.data() + sizeof(A) + sizeof(P) + sizeof(int), &g->n, sizeof(int64_t));
+    if (g->bfs_exec && g->bfs_key != key) {
+      (void)hipGraphExecDestroy(g->bfs_exec);
+      g->bfs_exec = nullptr;
+    }
+    if (!g->bfs_exec) {
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+      if (e == hipSuccess) {
+        for (int k = 0; k < kReplayRounds; k++) level_round(A, blocks, g->n, P, st);
+        const hipError_t ec = hipStreamEndCapture(st, &graph);
+        e = ec != hipSuccess ? ec : hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&g->bfs_exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        g->bfs_exec = nullptr;
+        return hip_fail(e, "bfs round graph", __FILE__, __LINE__);
+      }
+      g->bfs_key = key;
+    }
+    for (int64_t r = 0;; r += kReplayRounds) {
+      CG_HIP(hipGraphLaunch(g->bfs_exec, st));
+      CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+      CG_HIP(hipStreamSynchronize(st));
+      if (hc.done || r > o.max_levels + kReplayRounds) break;
+    }
   } else {
     // Rounds are enqueued in chunks without host synchronisation; kernels of
     // rounds past the end see ctrl->done and return at once.  The host polls
@@ -1354,11 +1412,11 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
           rev[nrev++] = rb;
           hipEventRecord(ra, st);
         }
-        bfs_level<<<blocks, kBlock, 0, st>>>(A, (int)d);
+        bfs_level<<<blocks, kBlock, 0, st>>>(A);
         bfs_push_scan<<<1, 1024, 0, st>>>(A);
-        bfs_push_big<<<blocks, kBlock, 0, st>>>(A, (int)d);
+        bfs_push_big<<<blocks, kBlock, 0, st>>>(A);
         if (rb) hipEventRecord(rb, st);
-        bfs_decide<<<1, 64, 0, st>>>(A, (int)d, g->n, P);
+        bfs_decide<<<1, 64, 0, st>>>(A, g->n, P);
       }
       CG_HIP(hipGetLastError());
       CG_HIP(hipMemcpyAsync(&hc, A.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
@@ -1379,33 +1437,25 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
     int64_t nh = hc.rounds < kHistCap ? hc.rounds : kHistCap;
     if (stats->max_hist < nh) nh = stats->max_hist;
     if (nh > 0) {
-      // history slices straight into the caller's arrays (long long and
-      // int64_t, int and int32_t have the same layout here)
-      const size_t b8 = sizeof(long long) * (size_t)nh;
-      if (stats->nf_history) CG_HIP(hipMemcpy(stats->nf_history, A.hist->nf, b8, hipMemcpyDeviceToHost));
-      if (stats->pairs_found) CG_HIP(hipMemcpy(stats->pairs_found, A.hist->nf, b8, hipMemcpyDeviceToHost));
-      if (stats->edges_scanned)
-        CG_HIP(hipMemcpy(stats->edges_scanned, A.hist->scanned, b8, hipMemcpyDeviceToHost));
-      if (stats->rows_scanned) CG_HIP(hipMemcpy(stats->rows_scanned, A.hist->rows, b8, hipMemcpyDeviceToHost));
-      if (stats->direction_history)
-        CG_HIP(hipMemcpy(stats->direction_history, A.hist->dir, sizeof(int) * nh, hipMemcpyDeviceToHost));
-      if (stats->level_ms) {
-        // the decide times land in the caller's array as raw clock values
-        // first; each is turned into its round's span in place
-        CG_HIP(hipMemcpy(stats->level_ms, A.hist->t, sizeof(unsigned long long) * nh,
-                         hipMemcpyDeviceToHost));
-        unsigned long long prev = hc.t0;
-        for (int64_t r = 0; r < nh; r++) {
-          unsigned long long t = 0;
-          std::memcpy(&t, &stats->level_ms[r], sizeof(t));
-          double ms = (double)(t - prev) / hz * 1e3;
-          prev = t;
-          if (2 * r + 1 < nrev) {
-            float rm = 0;
-            if (hipEventElapsedTime(&rm, rev[2 * r], rev[2 * r + 1]) == hipSuccess) ms = rm;
-          }
-          stats->level_ms[r] = ms;
+      std::vector<HistRow> hr((size_t)nh);
+      CG_HIP(hipMemcpy(hr.data(), A.hist->r, sizeof(HistRow) * nh, hipMemcpyDeviceToHost));
+      unsigned long long prev = hc.t0;
+      for (int64_t r = 0; r < nh; r++) {
+        const HistRow &H = hr[(size_t)r];
+        if (stats->nf_history) stats->nf_history[r] = H.nf;
+        if (stats->pairs_found) stats->pairs_found[r] = H.nf;
+        if (stats->edges_scanned) stats->edges_scanned[r] = H.scanned;
+        if (stats->rows_scanned) stats->rows_scanned[r] = H.rows;
+        if (stats->direction_history) stats->direction_history[r] = H.dir;
+        // a round's span: decide to decide on the device clock, or its
+        // events (level kernels only) where they were recorded
+        double ms = (double)(H.t - prev) / hz * 1e3;
+        prev = H.t;
+        if (2 * r + 1 < nrev) {
+          float rm = 0;
+          if (hipEventElapsedTime(&rm, rev[2 * r], rev[2 * r + 1]) == hipSuccess) ms = rm;
         }
+        if (stats->level_ms) stats->level_ms[r] = ms;
       }
     }
     unsigned long long sc[2] = {0, 0};

@@ -66,6 +66,10 @@ struct DeviceGraph {
   void (*msdata_free)(void *) = nullptr;
   // Pinned host mailbox for the done flag polled between level chunks.
   int32_t *host_flag = nullptr;
+  // Single-root graph replay: the last instantiated chunk of level rounds
+  // and the launch arguments it was captured with (reused while they match).
+  hipGraphExec_t bfs_exec = nullptr;
+  std::vector<char> bfs_key;
 };
 
 int graph_finalize(DeviceGraph *g);  // computes m_weight, stream, mailbox

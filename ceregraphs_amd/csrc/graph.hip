@@ -292,6 +292,7 @@ static void graph_free(DeviceGraph *g) {
   if (g->out_ptr) hipFree(g->out_ptr);
   if (g->out_idx) hipFree(g->out_idx);
   if (g->m_weight) hipFree(g->m_weight);
+  if (g->bfs_exec) hipGraphExecDestroy(g->bfs_exec);
   if (g->ws1) hipFree(g->ws1);
   if (g->wsm) hipFree(g->wsm);
   if (g->msdata && g->msdata_free) g->msdata_free(g->msdata);

@@ -78,7 +78,7 @@ def main(argv=None):
                         None, r.stats))
   else:
     for s in sources:
-      r = dg.bfs(s, direction=direction)
+      r = dg.bfs(s, direction=direction, capture=True)  # level loop replayed from a HIP graph
       results.append((s, r.parents, r.levels, r.stats.total_ms, r.stats.output_ms,
                       r.stats.rounds_completed, r.stats))
 

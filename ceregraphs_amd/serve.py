@@ -212,7 +212,7 @@ def plain_loop(slots, stream, deadline_s=120.0):
     if stream.t[q] > now:
       time.sleep(stream.t[q] - now)
     g = int(stream.graph[q])
-    r = slots[g].dg.bfs(int(stream.root[q]), device_out=outs[g])
+    r = slots[g].dg.bfs(int(stream.root[q]), device_out=outs[g], capture=True)
     lat[q] = time.perf_counter() - t0 - stream.t[q]
     m[q] = r.stats.m_edges
   span = float(np.max(stream.t + lat) - stream.t[0])

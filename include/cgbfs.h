@@ -81,9 +81,11 @@ typedef struct {
                          else 8, multi-source: 2 below 2^19, 4 below 2^23,
                          else 8)                                            */
   int64_t max_levels; /* abort with CGBFS_ETIMEOUT beyond this        (n+1)  */
-  int32_t capture;    /* multi-source: capture the round kernels once into a
-                         HIP graph and replay it, deciding every level on the
-                         device (no per-round host poll, no per-round events) */
+  int32_t capture;    /* capture the round kernels once into a HIP graph and
+                         replay it, deciding every level on the device (no
+                         per-round host poll, no per-round events); the graph
+                         is kept on the handle while the call's arguments
+                         match.  Single root: the per-level loop only      (0) */
   int32_t seg_grab;   /* multi-source: 256-entry hub-row segments a wavefront
                          takes per grab, 1..1024                       (32)  */
   int32_t push_wave_degree; /* single root: a wavefront expands each frontier
@@ -208,7 +210,7 @@ CGBFS_API int cgbfs_bfs(cgbfs_graph *g, int64_t source, int32_t *parents, int32_
  * (code run under HIP graph capture, tight serving loops, harnesses that
  * forbid allocation).  No counterpart in the reference.  Its workspace (the
  * frontier bitmaps, big-vertex list, control block and round history,
- * 26,380,032 bytes) is one file-scope device array per GPU, held from library load
+ * 26,396,416 bytes) is one file-scope device array per GPU, held from library load
  * until the process ends and sized for graphs of at most 2^21 vertices:
  * larger graphs get CGBFS_EINVAL (use cgbfs_bfs).  parents/levels: int32[n]
  * device arrays on the graph's device.  Results and statistics equal

@@ -246,6 +246,34 @@ def test_msbfs_hub_segments_lowest_id():
       _check_ms_stats(r.stats, levs, d)
 
 
+def test_msbfs_push_side_table_full():
+  """Push rounds whose new pairs do not all fit the side table that defers
+  their parents: 64 roots each linked to the same 2000 vertices (every one
+  new in all 64 lanes in round 0: 2000 x 66 words, more than the 131072 the
+  table holds at this size), which in turn link to 4 of another 2000 (new
+  in all lanes in round 1, several frontier in-neighbours each, so the
+  lowest id decides).  Vertices whose entry does not fit take the direct
+  path; the outputs are the oracle's either way, every round a push or
+  not."""
+  rng = np.random.default_rng(5)
+  roots = np.arange(64)
+  mid = np.arange(64, 2064)
+  far = np.arange(2064, 4064)
+  e = [(int(r), int(m)) for r in roots for m in mid]
+  e += [(int(m), int(x)) for m in mid for x in rng.choice(far, 4, replace=False)]
+  g = _edges_graph(e, 4064, directed=False)
+  levs = np.stack([O.bfs(g, int(s))[0] for s in roots])
+  with _device(g) as dg:
+    for d, cap in [(d, False) for d in DIRECTIONS] + [("topdown", True), ("beamer", True)]:
+      r = dg.msbfs(roots, direction=d, capture=cap)
+      for l, s in enumerate(roots):
+        lev, par, _, _ = O.bfs(g, int(s))
+        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16),
+                                      err_msg=f"lane {l} dir {d} capture {cap}")
+        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"lane {l} dir {d} capture {cap}")
+      _check_ms_stats(r.stats, levs, d)
+
+
 def _check_ms_stats(st, levs, direction):
   """Round accounting of a multi-source pass against the lanes' oracle
   levels: round r reaches the (lane, vertex) pairs at level r+1, nf counts
@@ -405,10 +433,12 @@ def test_tuning_knobs_keep_results_and_are_checked():
       assert e.value.code == _lib.CGBFS_EINVAL, bad
 
 
-@pytest.mark.parametrize("loop", ["persistent", "kernels"])
-def test_level_loops_equal_oracle(loop):
+@pytest.mark.parametrize("loop,capture", [("persistent", False), ("kernels", False),
+                                          ("kernels", True)])
+def test_level_loops_equal_oracle(loop, capture):
   """Both ways of running the levels -- one cooperative launch with grid
-  barriers, or the fixed per-level launch sequence -- give the oracle's
+  barriers, or the fixed per-level launch sequence, enqueued or replayed
+  from a captured HIP graph -- give the oracle's
   levels, lowest-id parents, per-round counts and m, on undirected and
   directed graphs, under every direction policy, on a graph with a
   1000-level path (1000 rounds of barriers) and with max_levels hit."""
@@ -422,15 +452,42 @@ def test_level_loops_equal_oracle(loop):
     with DeviceGraph.from_rmat(scale, ef, seed, abcd=abcd, directed=directed) as dg:
       for s in (int(np.argmax(deg)), int(np.nonzero(deg)[0][-1]), 0):
         for d in DIRECTIONS:
-          _check_single(dg, g, s, d, loop=loop)
+          _check_single(dg, g, s, d, loop=loop, capture=capture)
   edges = [(i, i + 1) for i in range(1000)] + [(500, j) for j in range(1001, 1400)]
   g = _edges_graph(edges, 1500, directed=False)
   with _device(g) as dg:
     for d in DIRECTIONS:
-      _check_single(dg, g, 0, d, loop=loop)
+      _check_single(dg, g, 0, d, loop=loop, capture=capture)
     with pytest.raises(_lib.CgbfsError) as e:
-      dg.bfs(0, loop=loop, max_levels=10)
+      dg.bfs(0, loop=loop, max_levels=10, capture=capture)
     assert e.value.code == _lib.CGBFS_ETIMEOUT
+
+
+def test_bfs_replay_reuse_and_rekey():
+  """The captured level graph is kept on the handle and replayed for every
+  source while the call's arguments match, and captured again when they
+  change (other output buffers, host outputs, another direction): every
+  search still equals the oracle, and so does a plain search between them."""
+  import torch
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(15, 16, 6)
+  g = O.Graph(n, ip, ii, op, oi, True)
+  deg = np.diff(op)
+  bufs = [(torch.empty(n, dtype=torch.int32, device="cuda"),
+           torch.empty(n, dtype=torch.int32, device="cuda")) for _ in range(2)]
+  sources = [int(np.argmax(deg))] + [int(x) for x in np.nonzero(deg)[0][[1, 7, 100, -1]]]
+  with DeviceGraph.from_rmat(15, 16, 6) as dg:
+    for i, s in enumerate(sources * 2):
+      lev0, par0, hist, rounds = O.bfs(g, s)
+      for out in (bufs[i % 2], None):
+        for d in ("beamer", "topdown") if i % 3 else ("beamer",):
+          r = dg.bfs(s, direction=d, loop="kernels", capture=True, device_out=out)
+          lev = out[1].cpu().numpy() if out is not None else r.levels
+          par = out[0].cpu().numpy() if out is not None else r.parents
+          np.testing.assert_array_equal(lev, lev0, err_msg=f"src {s} dir {d} device {out is not None}")
+          np.testing.assert_array_equal(par, par0, err_msg=f"src {s} dir {d} device {out is not None}")
+          assert r.stats.rounds_completed == rounds
+          np.testing.assert_array_equal(r.stats.nf_history, hist)
+      _check_single(dg, g, s, "beamer", loop="kernels")
 
 
 def test_workgroup_loop_equal_oracle():
