@@ -42,6 +42,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -222,7 +223,9 @@ struct MsArgs {
   // first-run segments, later segments
   unsigned long long *cursor;
   const uint64_t *noin;  // vertices without in-edges, one bit each
-  int *host_done;        // pinned host flag raised with done (graph replay), or null
+  // Pinned host mailbox (graph replay), or null: [0] the done flag, raised
+  // last, [1] the status, [2..3] the rounds (int64).
+  int *host_done;
   // Vertex-state slot of each vertex (hub-first order) and the adjacency
   // arrays re-expressed in slots, entry for entry.  The frontier words fw,
   // which the pull gathers through in_slot, are indexed by slot, so the
@@ -1780,7 +1783,12 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   if (done) {
     st_ctrl(&c->listed, 1);  // nothing left to clear
     st_ctrl(&c->done, 1);
-    if (A.host_done) __hip_atomic_store(A.host_done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (A.host_done) {  // the mailbox: status and rounds, then the flag
+      __hip_atomic_store(A.host_done + 1, ld_ctrl(&c->status), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(reinterpret_cast<long long *>(A.host_done + 2), (long long)(d + 1),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(A.host_done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
@@ -2281,8 +2289,15 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
       e = hipGraphLaunch(md->exec, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && *flag) {  // the mailbox holds what the host needs
+      std::atomic_thread_fence(std::memory_order_acquire);
+      hc.done = 1;
+      hc.status = flag[1];
+      hc.rounds = (long long)(((uint64_t)(uint32_t)flag[3] << 32) | (uint32_t)flag[2]);
+    } else {
+      if (e == hipSuccess) e = hipMemcpyAsync(&hc, A.ctrl, sizeof(MsCtrl), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
     A.host_done = nullptr;
   } else {
     // One host poll per round: only the parts that are on are launched, and
