@@ -376,9 +376,10 @@ __device__ __forceinline__ void push_rows_packed(const LevelArgs &A, int32_t *rm
 // the degrees of bigq[0..j), bigpre[nb] = big_edges.
 __global__ void __launch_bounds__(1024) bfs_push_scan(LevelArgs A) {
   Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
+  // the gate's words loaded together, then tested
+  const int done = ld_ctrl(&c->done), dir = ld_ctrl(&c->dir);
   const int64_t nb = (int64_t)ld_ctrl(&c->big_tail);
-  if (nb == 0) return;
+  if (done || dir != 0 || nb == 0) return;
   __shared__ long long wsum[1024 / kWave];
   __shared__ long long carry;
   if (threadIdx.x == 0) carry = 0;
@@ -422,10 +423,10 @@ __global__ void __launch_bounds__(1024) bfs_push_scan(LevelArgs A) {
 // over the degree prefix staged in LDS (kBigLds entries per pass).
 __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A) {
   Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done) || ld_ctrl(&c->dir) != 0) return;
+  const int done = ld_ctrl(&c->done), dir = ld_ctrl(&c->dir);
   const int d = (int)ld_ctrl(&c->rounds);
   const int64_t nb = (int64_t)ld_ctrl(&c->big_tail);
-  if (nb == 0) return;
+  if (done || dir != 0 || nb == 0) return;
   __shared__ unsigned long long red[kWavesPerBlock];
   __shared__ long long pre[kBigLds + 1];
   unsigned long long mf = 0, in_sum = 0, scanned = 0, rows = 0, found = 0;
@@ -709,18 +710,26 @@ __device__ __forceinline__ int next_direction(const BfsParams &P, int cur, long 
   }
 }
 
-__device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsParams &P,
-                             long long nf) {
+// Every control word the decide reads, loaded in one batch: the loads are
+// independent, so they are in flight together instead of one after another.
+__device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsParams &P) {
   Ctrl *c = A.ctrl;
+  const long long nf = (long long)ld_ctrl(&c->nf);
   const int cur = ld_ctrl(&c->dir);
   const long long mf = (long long)ld_ctrl(&c->mf);
-  const long long mu = ld_ctrl(&c->mu) - (long long)ld_ctrl(&c->in_sum);
+  const long long mu0 = ld_ctrl(&c->mu);
+  const long long in_sum = (long long)ld_ctrl(&c->in_sum);
+  const long long scanned = (long long)ld_ctrl(&c->scanned);
+  const long long rows = (long long)ld_ctrl(&c->rows);
+  const long long prev_nf = ld_ctrl(&c->prev_nf);
+  const long long seen0 = ld_ctrl(&c->seen);
+  const long long mu = mu0 - in_sum;
   unsigned long long now = wall_clock64();
   if (d < kHistCap) {
     HistRow &H = A.hist->r[d];
     H.nf = nf;
-    H.scanned = (long long)ld_ctrl(&c->scanned);
-    H.rows = (long long)ld_ctrl(&c->rows);
+    H.scanned = scanned;
+    H.rows = rows;
     H.dir = cur;
     H.t = now;
   }
@@ -738,11 +747,11 @@ __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsPara
     st_ctrl(&c->done, 1);
     return;
   }
-  const int dir = next_direction(P, cur, nf, mf, mu, ld_ctrl(&c->prev_nf), n);
+  const int dir = next_direction(P, cur, nf, mf, mu, prev_nf, n);
   st_ctrl(&c->prev_nf, nf);
   // The next pull visits every chunk; once fewer vertices with in-edges are
   // unreached than there are chunks, most chunks have nothing to do.
-  const long long seen = ld_ctrl(&c->seen) + nf;
+  const long long seen = seen0 + nf;
   st_ctrl(&c->seen, seen);
   st_ctrl(&c->light, P.n_in - seen < (long long)(A.nwords / 2) ? 1 : 0);
   st_ctrl(&c->dir, dir);
@@ -758,8 +767,10 @@ __device__ void decide_round(const LevelArgs &A, int d, int64_t n, const BfsPara
 __global__ void bfs_decide(LevelArgs A, int64_t n, BfsParams P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done)) return;
-  decide_round(A, (int)ld_ctrl(&c->rounds), n, P, (long long)ld_ctrl(&c->nf));
+  const int done = ld_ctrl(&c->done);
+  const int d = (int)ld_ctrl(&c->rounds);
+  if (done) return;
+  decide_round(A, d, n, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -1078,9 +1089,10 @@ __device__ void push_big_rows(const LevelArgs &A, int d) {
 // dispatch less than a kernel per direction.
 __global__ void __launch_bounds__(kBlock, 8) bfs_level(LevelArgs A) {
   const Ctrl *c = A.ctrl;
-  if (ld_ctrl(&c->done)) return;
+  const int done = ld_ctrl(&c->done), dir = ld_ctrl(&c->dir);
   const int d = (int)ld_ctrl(&c->rounds);
-  if (ld_ctrl(&c->dir) == 0) push_level_bitmap(A, d);
+  if (done) return;
+  if (dir == 0) push_level_bitmap(A, d);
   else pull_level(A, d);
 }
 
@@ -1166,7 +1178,7 @@ __global__ void __launch_bounds__(kBlock, 8) bfs_persistent(LevelArgs A, int64_t
       pull_level(A, (int)d);
     }
     if (!grid_barrier(c, gen)) break;
-    if (blockIdx.x == 0 && threadIdx.x == 0) decide_round(A, (int)d, n, P, (long long)ld_ctrl(&c->nf));
+    if (blockIdx.x == 0 && threadIdx.x == 0) decide_round(A, (int)d, n, P);
     if (!grid_barrier(c, gen)) break;
   }
 }
