@@ -1326,6 +1326,27 @@ __device__ void compact_part(const MsArgs &A, const RoundState &R, const int32_t
 // This round's active segments: those of the big vertices that still need
 // a pull lane (which = 0, in view) or that are on the push lanes' frontier
 // (which = 1, out view).  One wavefront-aggregated append per wavefront.
+// Writes each lane's run of consecutive segment ids (and, where needs is
+// given, its lane mask) at its place in a list, the whole wavefront writing
+// one lane's run at a time: a hub's run is thousands of segments, which
+// its own lane alone would write one by one while the others wait.
+__device__ __forceinline__ void list_runs(int32_t *ids, uint64_t *needs, unsigned long long base,
+                                          int32_t first, int32_t cnt, uint64_t need) {
+  const int lane = lane_id();
+  uint64_t owners = ballot64(cnt > 0);
+  while (owners) {
+    const int o = __ffsll((unsigned long long)owners) - 1;
+    owners &= owners - 1;
+    const unsigned long long ob = (unsigned long long)shfl64((uint64_t)base, o);
+    const int32_t of = __shfl(first, o), oc = __shfl(cnt, o);
+    const uint64_t on = shfl64(need, o);
+    for (int32_t k = lane; k < oc; k += kWave) {
+      ids[ob + k] = of + k;
+      if (needs) needs[ob + k] = on;
+    }
+  }
+}
+
 __device__ void active_segs_part(const MsArgs &A, const RoundState &R, int which) {
   const uint64_t mask = which ? R.qm : R.pm;
   const int d = R.d;
@@ -1359,9 +1380,12 @@ __device__ void active_segs_part(const MsArgs &A, const RoundState &R, int which
       unsigned long long base = 0;
       if (lane == 0) base = atomicAdd(tail, (unsigned long long)total);
       base = __shfl(base, 0) + (unsigned long long)(incl - c0);
-      for (int32_t k = 0; k < c0; k++) out[base + k] = first + k;
-      if (!which)
+      if (!which) {  // at most kSegRun each: every lane writes its own
+        for (int32_t k = 0; k < c0; k++) out[base + k] = first + k;
         for (int32_t k = 0; k < c0; k++) A.segneed[base + k] = need;
+      } else {
+        list_runs(out, nullptr, base, first, c0, 0ull);
+      }
     }
     if (which) continue;
     const int32_t incl1 = wave_scan_add(c1);
@@ -1370,10 +1394,7 @@ __device__ void active_segs_part(const MsArgs &A, const RoundState &R, int which
       unsigned long long base = 0;
       if (lane == 0) base = atomicAdd(&A.ctrl->segtail_late, (unsigned long long)total1);
       base = __shfl(base, 0) + (unsigned long long)(incl1 - c1);
-      for (int32_t k = 0; k < c1; k++) {
-        A.segq_late[base + k] = first + c0 + k;
-        A.segneed_late[base + k] = need;
-      }
+      list_runs(A.segq_late, A.segneed_late, base, first + c0, c1, need);
     }
   }
 }
