@@ -232,6 +232,7 @@ struct LevelArgs {
   const int32_t *out_idx;
   const int64_t *in_ptr;
   const int32_t *in_idx;
+  const int2 *in_head;  // first two entries of every in-row (-1 past the end)
   int32_t *levels;
   uint32_t *parents;
   uint32_t *visited;
@@ -472,19 +473,21 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A) {
 // Pull round
 // ---------------------------------------------------------------------------
 // What a chunk needs before its rows can be scanned: its two visited words
-// and, in a dense round, the row offsets of all 64 vertices.  Loaded one
-// chunk ahead, so those loads are in flight while the chunk before is being
-// scanned and a chunk's scan starts with its first row entries.
+// and, in a dense round, the row offsets and row heads (first two entries)
+// of all 64 vertices.  Loaded one chunk ahead, so those loads are in flight
+// while the chunk before is being scanned, and a chunk's scan starts with
+// the frontier bits of its row heads.
 struct PullPre {
   int32_t ch;       // chunk, -1 for none
   uint32_t vlo, vhi;
   int64_t beg, end;
+  int2 hd;
 };
 
-// A dense round (most vertices unvisited) loads every lane's offsets with
-// the visited words, so the two loads overlap; a light round (ctrl->light)
-// loads offsets only for unvisited vertices, once their bits are known,
-// since almost every vertex is visited there.
+// A dense round (most vertices unvisited) loads every lane's offsets and
+// row head with the visited words, so the loads overlap; a light round
+// (ctrl->light) loads them only for unvisited vertices, once their bits are
+// known, since almost every vertex is visited there.
 __device__ __forceinline__ void pull_prefetch(const LevelArgs &A, int32_t ch, bool light,
                                               PullPre &p) {
   p.ch = ch;
@@ -497,6 +500,7 @@ __device__ __forceinline__ void pull_prefetch(const LevelArgs &A, int32_t ch, bo
     const int64_t vv = v < A.n ? v : A.n - 1;
     p.beg = A.in_ptr[vv];
     p.end = A.in_ptr[vv + 1];
+    p.hd = A.in_head[vv];
   }
 }
 
@@ -531,26 +535,33 @@ __device__ void pull_level(const LevelArgs &A, int d) {
     bool todo = v < A.n && !((myword >> (lane & 31)) & 1u);
     if (!__any(todo)) return;
     int64_t beg, end;
+    int2 hd;
     if (light) {
       beg = todo ? A.in_ptr[v] : 0;
       end = todo ? A.in_ptr[v + 1] : 0;
+      hd = A.in_head[todo ? v : 0];
     } else {
       beg = todo ? p.beg : 0;
       end = todo ? p.end : 0;
+      hd = p.hd;
     }
     bool hit = false;
     int32_t par = -1;
     if (todo) rows++;
     // Head probe: every row's first kPullUnroll entries at once, one lane
-    // per row (their index loads, then their frontier-bit loads, in flight
-    // together); the first hit in row order is the lowest id.  Most rows of
-    // a dense level end here.  Lanes past the row end reread its first entry.
+    // per row (their frontier-bit loads in flight together); the first hit
+    // in row order is the lowest id.  Most rows of a dense level end here.
+    // The first two entries come from the row heads, loaded in order with
+    // the offsets; entries past the row end are not tested.
     const int64_t hend = end - beg < kPullUnroll ? end : beg + kPullUnroll;
     if (todo && beg < end) {
       int32_t u[kPullUnroll];
       uint32_t fw[kPullUnroll];
 #pragma unroll
-      for (int k = 0; k < kPullUnroll; k++) u[k] = A.in_idx[beg + k < hend ? beg + k : beg];
+      for (int k = 0; k < kPullUnroll; k++) {
+        const int32_t h = k == 0 ? hd.x : k == 1 ? hd.y : A.in_idx[beg + k < hend ? beg + k : beg];
+        u[k] = beg + k < hend ? h : hd.x;
+      }
 #pragma unroll
       for (int k = 0; k < kPullUnroll; k++) fw[k] = fcur[u[k] >> 5];
 #pragma unroll
@@ -1313,6 +1324,7 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
   A.out_idx = g->out_idx;
   A.in_ptr = g->in_ptr;
   A.in_idx = g->in_idx;
+  A.in_head = g->in_head;
   A.levels = lev;
   A.parents = par;
   A.visited = ws.visited;

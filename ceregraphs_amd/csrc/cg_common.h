@@ -51,6 +51,11 @@ struct DeviceGraph {
   // undirected_dedup rule) or out-degree (directed_source_visited),
   // bfs/implementation/bfs_timing.py:390-395.
   int32_t *m_weight = nullptr;
+  // Row heads of the in view: the first two in-neighbours of every vertex
+  // (-1 past the row end), 8 bytes per vertex in vertex order.  A pull's
+  // head probe reads them with the row offsets, in order, instead of one
+  // scattered line of in_idx per row.
+  int2 *in_head = nullptr;
   int64_t total_out_edges = 0;
   int64_t n_in = 0;  // vertices with at least one in-edge
   hipStream_t stream = nullptr;

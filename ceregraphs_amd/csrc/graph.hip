@@ -202,6 +202,16 @@ __global__ void m_weight_kernel(const int64_t *__restrict__ in_ptr,
   w[v] = (int32_t)(lo - b);
 }
 
+// The first two entries of every in-row (-1 past the row end).
+__global__ void row_head_kernel(const int64_t *__restrict__ in_ptr,
+                                const int32_t *__restrict__ in_idx, int64_t n,
+                                int2 *__restrict__ head) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int64_t b = in_ptr[v], e = in_ptr[v + 1];
+  head[v] = make_int2(b < e ? in_idx[b] : -1, b + 1 < e ? in_idx[b + 1] : -1);
+}
+
 // Counts the vertices with at least one in-edge (one atomic per wavefront).
 __global__ void count_in_kernel(const int64_t *__restrict__ in_ptr, int64_t n,
                                 unsigned long long *__restrict__ count) {
@@ -258,9 +268,13 @@ static int sort_unique_to_csr(uint64_t *keys, uint64_t *scratch, int64_t count, 
 int graph_finalize(DeviceGraph *g) {
   if (!g->stream) CG_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   CG_HIP(hipMalloc(&g->m_weight, sizeof(int32_t) * (g->n > 0 ? g->n : 1)));
+  CG_HIP(hipMalloc(&g->in_head, sizeof(int2) * (g->n > 0 ? g->n : 1)));
   if (g->n > 0) {
     m_weight_kernel<<<(unsigned)((g->n + 255) / 256), 256, 0, g->stream>>>(
         g->in_ptr, g->in_idx, g->out_ptr, g->n, g->symmetric, g->m_weight);
+    CG_HIP(hipGetLastError());
+    row_head_kernel<<<(unsigned)((g->n + 255) / 256), 256, 0, g->stream>>>(g->in_ptr, g->in_idx,
+                                                                          g->n, g->in_head);
     CG_HIP(hipGetLastError());
   }
   CG_HIP(hipHostMalloc((void **)&g->host_flag, 64, hipHostMallocDefault));
@@ -292,6 +306,7 @@ static void graph_free(DeviceGraph *g) {
   if (g->out_ptr) hipFree(g->out_ptr);
   if (g->out_idx) hipFree(g->out_idx);
   if (g->m_weight) hipFree(g->m_weight);
+  if (g->in_head) hipFree(g->in_head);
   if (g->bfs_exec) hipGraphExecDestroy(g->bfs_exec);
   if (g->ws1) hipFree(g->ws1);
   if (g->wsm) hipFree(g->wsm);
