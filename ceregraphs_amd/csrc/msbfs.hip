@@ -994,7 +994,7 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
   // First hits of the current chunk, as offsets into the row (rows here are
   // shorter than kSeg), indexed [lane bit][row] so the per-bit flush below
   // reads one conflict-free row of LDS and stores 64 consecutive vertices.
-  __shared__ TileOff ptile[kMsWaves][kWave][kWave];
+  __shared__ __align__(16) TileOff ptile[kMsWaves][kWave][kWave];
   TileOff (*pt)[kWave] = ptile[threadIdx.x / kWave];
   // Row marks of the current pass, [window][position].
   __shared__ int32_t rmap[kMsWaves][kUnroll][kWave];
@@ -1043,6 +1043,17 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
       bid = A.pin.chunk_big[ch] + (int32_t)__popcll(bigm & ((1ull << lane) - 1ull));
     const int32_t len = need ? (int32_t)(re0 - rb0) : 0;  // < kSeg for non-big rows
     if (len) rows++;
+    // First-hit offsets start at 0 (the row's first entry) for every lane
+    // bit and row: what the head probe finds needs no tile write, only the
+    // hits of the scan further in do.  Each lane clears one lane bit's row
+    // of the tile; the wavefront's LDS operations stay in order, so the
+    // previous chunk's flush has read it.
+    {
+      uint4 *trow = reinterpret_cast<uint4 *>(pt[lane]);
+#pragma unroll
+      for (int j = 0; j < (int)(kWave * sizeof(TileOff) / sizeof(uint4)); j++)
+        trow[j] = make_uint4(0u, 0u, 0u, 0u);
+    }
     // Head probe: the first entry of every row that needs lanes, all at
     // once.  In a late round most rows end here: a row's lowest in-neighbours
     // are hubs, reached early.  The rows still open are then laid end to end
@@ -1055,12 +1066,6 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
       const uint64_t hw = fr[hok ? hs : 0];
       got = hok ? hw & need : 0ull;
       if (len) scanned++;
-      uint64_t m = got;
-      while (m) {
-        const int l = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        pt[l][lane] = 0;
-      }
     }
     const int32_t lenr = (len && (need & ~got)) ? len - 1 : 0;  // entries left to scan
     const int64_t rbr = rb + 1;
@@ -1278,24 +1283,41 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A, int late) {
   // a later one of the same vertex.
   int32_t prev_bid = -1, prev_i = -1;
   uint64_t prev_got = 0;
-  while (feed.next(gb, ge)) for (int32_t i = gb; i < ge; i++) {
-    const int32_t s = segs[i];
-    uint64_t need = needs[i];
-    const int32_t bid = A.pin.seg_bid[s];
-    if (bid == prev_bid && i > prev_i) need &= ~prev_got;
-    else prev_got = 0;
-    prev_bid = bid;
-    prev_i = i;
-    const int64_t beg = A.pin.seg_beg[s];
-    if (late && need) {
-      const uint32_t low = A.cand[(int64_t)bid * 64 + lane];
-      need &= ~ballot64(low < (uint32_t)A.in_idx[beg]);
+  while (feed.next(gb, ge)) for (int32_t b0 = gb; b0 < ge; b0 += kWave) {
+    // The descriptors of up to 64 segments of the grab, one per lane, are
+    // loaded together (their list entry, then vertex, start and length, and
+    // in the second launch the first id), so a segment's scan starts with
+    // its slots instead of two dependent loads; the loop reads them from
+    // their lanes.
+    const int32_t nb = ge - b0 < kWave ? ge - b0 : kWave;
+    const int32_t ii = b0 + (lane < nb ? lane : 0);
+    const int32_t s_l = segs[ii];
+    const uint64_t need_l = needs[ii];
+    const int32_t bid_l = A.pin.seg_bid[s_l];
+    const int64_t beg_l = A.pin.seg_beg[s_l];
+    const int32_t len_l = A.pin.seg_len[s_l];
+    const int32_t first_l = late ? A.in_idx[beg_l] : 0;
+    for (int32_t j = 0; j < nb; j++) {
+      const int32_t i = b0 + j;
+      uint64_t need = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(need_l >> 32), j) << 32) |
+                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)need_l, j);
+      const int32_t bid = __builtin_amdgcn_readlane(bid_l, j);
+      if (bid == prev_bid && i > prev_i) need &= ~prev_got;
+      else prev_got = 0;
+      prev_bid = bid;
+      prev_i = i;
+      const int64_t beg = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(beg_l >> 32), j) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)beg_l, j));
+      if (late && need) {
+        const uint32_t low = A.cand[(int64_t)bid * 64 + lane];
+        need &= ~ballot64(low < (uint32_t)__builtin_amdgcn_readlane(first_l, j));
+      }
+      if (!need) continue;
+      const int64_t end = beg + __builtin_amdgcn_readlane(len_l, j);
+      uint64_t gj = ms_seg_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], R.use_fb, scanned);
+      prev_got |= gj;
+      if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
     }
-    if (!need) continue;
-    const int64_t end = beg + A.pin.seg_len[s];
-    uint64_t gj = ms_seg_pull(A, fr, beg, end, need, &A.cand[(int64_t)bid * 64], R.use_fb, scanned);
-    prev_got |= gj;
-    if (gj && lane == 0) atomicOr((unsigned long long *)&A.bgot[bid], (unsigned long long)gj);
   }
   const unsigned long long t = ms_block_sum(scanned, red);
   if (threadIdx.x == 0 && t) atomicAdd(&c->pscanned, t);
