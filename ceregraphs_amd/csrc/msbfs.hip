@@ -58,7 +58,7 @@ constexpr int64_t kSeg = 256;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
 constexpr int kUnroll = 2;          // pull steps whose loads are issued together
 #ifndef CG_MS_FLUSH
-#define CG_MS_FLUSH 16
+#define CG_MS_FLUSH 4
 #endif
 constexpr int kFlush = CG_MS_FLUSH;  // lane bits per batch of the pull's output flush
 constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay mode)
@@ -69,7 +69,7 @@ constexpr int32_t kGrabDense = CG_MS_DENSE_GRAB;  // chunks per grab of a dense 
 // Waves per SIMD the chunk pull is compiled for (its register budget is
 // 512 / kMsPullWaves VGPRs).
 #ifndef CG_MS_PULL_WAVES
-#define CG_MS_PULL_WAVES 7
+#define CG_MS_PULL_WAVES 6
 #endif
 constexpr int kMsPullWaves = CG_MS_PULL_WAVES;
 #ifndef CG_MS_PUSH_FILTER
@@ -722,6 +722,13 @@ __device__ __forceinline__ void tally_lds_init(unsigned long long *lds) {
 }
 
 
+// Lane j's 64-bit value (j wavefront-uniform).
+__device__ __forceinline__ int64_t readlane64(int64_t x, int j) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), j);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 constexpr int kSegSteps = (int)(kSeg / kWave);
 static_assert(kSeg % kWave == 0, "a segment is a whole number of wavefront steps");
 
@@ -1299,15 +1306,13 @@ __global__ void __launch_bounds__(kMsBlock) ms_pull_segs(MsArgs A, int late) {
     const int32_t first_l = late ? A.in_idx[beg_l] : 0;
     for (int32_t j = 0; j < nb; j++) {
       const int32_t i = b0 + j;
-      uint64_t need = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(need_l >> 32), j) << 32) |
-                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)need_l, j);
+      uint64_t need = (uint64_t)readlane64((int64_t)need_l, j);
       const int32_t bid = __builtin_amdgcn_readlane(bid_l, j);
       if (bid == prev_bid && i > prev_i) need &= ~prev_got;
       else prev_got = 0;
       prev_bid = bid;
       prev_i = i;
-      const int64_t beg = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(beg_l >> 32), j) << 32) |
-                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)beg_l, j));
+      const int64_t beg = readlane64(beg_l, j);
       if (late && need) {
         const uint32_t low = A.cand[(int64_t)bid * 64 + lane];
         need &= ~ballot64(low < (uint32_t)__builtin_amdgcn_readlane(first_l, j));
