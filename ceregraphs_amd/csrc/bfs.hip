@@ -233,6 +233,9 @@ struct LevelArgs {
   const int64_t *in_ptr;
   const int32_t *in_idx;
   const int2 *in_head;  // first two entries of every in-row (-1 past the end)
+  // vertices without in-edges, as the visited bitmap's words: a search
+  // starts with them marked, so no pull looks at them again
+  const uint32_t *noin;
   int32_t *levels;
   uint32_t *parents;
   uint32_t *visited;
@@ -261,7 +264,7 @@ __global__ void __launch_bounds__(kBlock) bfs_init_fill(LevelArgs A) {
     A.parents[v] = kParentNone;
   }
   for (int64_t w = tid; w < A.nwords; w += stride) {
-    A.visited[w] = 0;
+    A.visited[w] = A.noin[w];
     A.fb[0][w] = 0;
     A.fb[1][w] = 0;
     A.fb[2][w] = 0;
@@ -1325,6 +1328,7 @@ static int bfs_run(DeviceGraph *g, const Workspace &ws, int64_t source, int32_t 
   A.in_ptr = g->in_ptr;
   A.in_idx = g->in_idx;
   A.in_head = g->in_head;
+  A.noin = reinterpret_cast<const uint32_t *>(g->noin);
   A.levels = lev;
   A.parents = par;
   A.visited = ws.visited;

@@ -56,6 +56,9 @@ struct DeviceGraph {
   // head probe reads them with the row offsets, in order, instead of one
   // scattered line of in_idx per row.
   int2 *in_head = nullptr;
+  // One bit per vertex without in-edges ([ceil(n/64)] words, bit v & 63 of
+  // word v >> 6): no search reaches such a vertex unless it is the source.
+  uint64_t *noin = nullptr;
   int64_t total_out_edges = 0;
   int64_t n_in = 0;  // vertices with at least one in-edge
   hipStream_t stream = nullptr;

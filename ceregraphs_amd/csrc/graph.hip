@@ -212,6 +212,14 @@ __global__ void row_head_kernel(const int64_t *__restrict__ in_ptr,
   head[v] = make_int2(b < e ? in_idx[b] : -1, b + 1 < e ? in_idx[b + 1] : -1);
 }
 
+// The bitmap of vertices without in-edges, one 64-bit word per wavefront.
+__global__ void noin_kernel(const int64_t *__restrict__ in_ptr, int64_t n,
+                            uint64_t *__restrict__ noin) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t m = __ballot(v < n && in_ptr[v + 1] == in_ptr[v]);
+  if (lane_id() == 0 && v < n) noin[v >> 6] = m;
+}
+
 // Counts the vertices with at least one in-edge (one atomic per wavefront).
 __global__ void count_in_kernel(const int64_t *__restrict__ in_ptr, int64_t n,
                                 unsigned long long *__restrict__ count) {
@@ -269,12 +277,15 @@ int graph_finalize(DeviceGraph *g) {
   if (!g->stream) CG_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   CG_HIP(hipMalloc(&g->m_weight, sizeof(int32_t) * (g->n > 0 ? g->n : 1)));
   CG_HIP(hipMalloc(&g->in_head, sizeof(int2) * (g->n > 0 ? g->n : 1)));
+  CG_HIP(hipMalloc(&g->noin, sizeof(uint64_t) * (size_t)((g->n + 63) / 64 + 1)));
   if (g->n > 0) {
     m_weight_kernel<<<(unsigned)((g->n + 255) / 256), 256, 0, g->stream>>>(
         g->in_ptr, g->in_idx, g->out_ptr, g->n, g->symmetric, g->m_weight);
     CG_HIP(hipGetLastError());
     row_head_kernel<<<(unsigned)((g->n + 255) / 256), 256, 0, g->stream>>>(g->in_ptr, g->in_idx,
                                                                           g->n, g->in_head);
+    CG_HIP(hipGetLastError());
+    noin_kernel<<<(unsigned)((g->n + 255) / 256), 256, 0, g->stream>>>(g->in_ptr, g->n, g->noin);
     CG_HIP(hipGetLastError());
   }
   CG_HIP(hipHostMalloc((void **)&g->host_flag, 64, hipHostMallocDefault));
@@ -307,6 +318,7 @@ static void graph_free(DeviceGraph *g) {
   if (g->out_idx) hipFree(g->out_idx);
   if (g->m_weight) hipFree(g->m_weight);
   if (g->in_head) hipFree(g->in_head);
+  if (g->noin) hipFree(g->noin);
   if (g->bfs_exec) hipGraphExecDestroy(g->bfs_exec);
   if (g->ws1) hipFree(g->ws1);
   if (g->wsm) hipFree(g->wsm);
