@@ -40,7 +40,7 @@ constexpr int kBigLds = 4096;         // push: big-list prefix entries staged pe
 constexpr int64_t kWaveDegree = 256;
 constexpr int64_t kSpanCap = (int64_t)1 << 24;  // push: rows laid end to end stay below this
 #ifndef CG_PULL_HEAD
-#define CG_PULL_HEAD 2
+#define CG_PULL_HEAD CG_ROW_HEAD
 #endif
 #ifndef CG_PULL_PASS_STEPS
 #define CG_PULL_PASS_STEPS 1
@@ -232,7 +232,7 @@ struct LevelArgs {
   const int32_t *out_idx;
   const int64_t *in_ptr;
   const int32_t *in_idx;
-  const int2 *in_head;  // first two entries of every in-row (-1 past the end)
+  const RowHead *in_head;  // first kRowHead entries of every in-row (-1 past the end)
   // vertices without in-edges, as the visited bitmap's words: a search
   // starts with them marked, so no pull looks at them again
   const uint32_t *noin;
@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(kBlock) bfs_push_big(LevelArgs A) {
 // Pull round
 // ---------------------------------------------------------------------------
 // What a chunk needs before its rows can be scanned: its two visited words
-// and, in a dense round, the row offsets and row heads (first two entries)
+// and, in a dense round, the row offsets and row heads (first kRowHead entries)
 // of all 64 vertices.  Loaded one chunk ahead, so those loads are in flight
 // while the chunk before is being scanned, and a chunk's scan starts with
 // the frontier bits of its row heads.
@@ -484,7 +484,7 @@ struct PullPre {
   int32_t ch;       // chunk, -1 for none
   uint32_t vlo, vhi;
   int64_t beg, end;
-  int2 hd;
+  RowHead hd;
 };
 
 // A dense round (most vertices unvisited) loads every lane's offsets and
@@ -538,7 +538,7 @@ __device__ void pull_level(const LevelArgs &A, int d) {
     bool todo = v < A.n && !((myword >> (lane & 31)) & 1u);
     if (!__any(todo)) return;
     int64_t beg, end;
-    int2 hd;
+    RowHead hd;
     if (light) {
       beg = todo ? A.in_ptr[v] : 0;
       end = todo ? A.in_ptr[v + 1] : 0;
@@ -554,7 +554,7 @@ __device__ void pull_level(const LevelArgs &A, int d) {
     // Head probe: every row's first kPullUnroll entries at once, one lane
     // per row (their frontier-bit loads in flight together); the first hit
     // in row order is the lowest id.  Most rows of a dense level end here.
-    // The first two entries come from the row heads, loaded in order with
+    // The first kRowHead entries come from the row heads, loaded in order with
     // the offsets; entries past the row end are not tested.
     const int64_t hend = end - beg < kPullUnroll ? end : beg + kPullUnroll;
     if (todo && beg < end) {
@@ -562,8 +562,8 @@ __device__ void pull_level(const LevelArgs &A, int d) {
       uint32_t fw[kPullUnroll];
 #pragma unroll
       for (int k = 0; k < kPullUnroll; k++) {
-        const int32_t h = k == 0 ? hd.x : k == 1 ? hd.y : A.in_idx[beg + k < hend ? beg + k : beg];
-        u[k] = beg + k < hend ? h : hd.x;
+        const int32_t h = k < kRowHead ? hd.id[k < kRowHead ? k : 0] : A.in_idx[beg + k < hend ? beg + k : beg];
+        u[k] = beg + k < hend ? h : hd.id[0];
       }
 #pragma unroll
       for (int k = 0; k < kPullUnroll; k++) fw[k] = fcur[u[k] >> 5];

@@ -37,6 +37,15 @@ int hip_fail(hipError_t err, const char *what, const char *file, int line);
       return ::cg::hip_fail(cg_err__, #call, __FILE__, __LINE__);      \
   } while (0)
 
+// The first kRowHead entries of an in-row, one vector load.
+#ifndef CG_ROW_HEAD
+#define CG_ROW_HEAD 2
+#endif
+constexpr int kRowHead = CG_ROW_HEAD;
+struct alignas(4 * kRowHead) RowHead {
+  int32_t id[kRowHead];
+};
+
 // Device-resident graph owned by one handle.
 struct DeviceGraph {
   int device = 0;
@@ -51,11 +60,11 @@ struct DeviceGraph {
   // undirected_dedup rule) or out-degree (directed_source_visited),
   // bfs/implementation/bfs_timing.py:390-395.
   int32_t *m_weight = nullptr;
-  // Row heads of the in view: the first two in-neighbours of every vertex
-  // (-1 past the row end), 8 bytes per vertex in vertex order.  A pull's
-  // head probe reads them with the row offsets, in order, instead of one
-  // scattered line of in_idx per row.
-  int2 *in_head = nullptr;
+  // Row heads of the in view: the first kRowHead in-neighbours of every
+  // vertex (-1 past the row end), in vertex order.  A pull's head probe
+  // reads them with the row offsets, in order, instead of one scattered
+  // line of in_idx per row.
+  RowHead *in_head = nullptr;
   // One bit per vertex without in-edges ([ceil(n/64)] words, bit v & 63 of
   // word v >> 6): no search reaches such a vertex unless it is the source.
   uint64_t *noin = nullptr;

@@ -202,14 +202,17 @@ __global__ void m_weight_kernel(const int64_t *__restrict__ in_ptr,
   w[v] = (int32_t)(lo - b);
 }
 
-// The first two entries of every in-row (-1 past the row end).
+// The first kRowHead entries of every in-row (-1 past the row end).
 __global__ void row_head_kernel(const int64_t *__restrict__ in_ptr,
                                 const int32_t *__restrict__ in_idx, int64_t n,
-                                int2 *__restrict__ head) {
+                                RowHead *__restrict__ head) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const int64_t b = in_ptr[v], e = in_ptr[v + 1];
-  head[v] = make_int2(b < e ? in_idx[b] : -1, b + 1 < e ? in_idx[b + 1] : -1);
+  RowHead h;
+#pragma unroll
+  for (int k = 0; k < kRowHead; k++) h.id[k] = b + k < e ? in_idx[b + k] : -1;
+  head[v] = h;
 }
 
 // The bitmap of vertices without in-edges, one 64-bit word per wavefront.
@@ -276,7 +279,7 @@ static int sort_unique_to_csr(uint64_t *keys, uint64_t *scratch, int64_t count, 
 int graph_finalize(DeviceGraph *g) {
   if (!g->stream) CG_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   CG_HIP(hipMalloc(&g->m_weight, sizeof(int32_t) * (g->n > 0 ? g->n : 1)));
-  CG_HIP(hipMalloc(&g->in_head, sizeof(int2) * (g->n > 0 ? g->n : 1)));
+  CG_HIP(hipMalloc(&g->in_head, sizeof(RowHead) * (g->n > 0 ? g->n : 1)));
   CG_HIP(hipMalloc(&g->noin, sizeof(uint64_t) * (size_t)((g->n + 63) / 64 + 1)));
   if (g->n > 0) {
     m_weight_kernel<<<(unsigned)((g->n + 255) / 256), 256, 0, g->stream>>>(
