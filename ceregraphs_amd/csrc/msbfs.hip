@@ -1174,48 +1174,56 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
     // big vertices with new lanes (rare after the round that reaches the hubs)
     const bool bgot_any = __any(bg && got);
     // Flush: per lane bit, 64 consecutive vertices' parents and levels.
-    uint64_t bits = wave_or64(got | nmiss | pl);
+    const uint64_t wr = got | nmiss | pl;  // the lane bits this vertex writes
+    uint64_t bits = wave_or64(wr);
     // First hits are the lowest ids with the bit, mostly near the row start.
     constexpr int kHead = 4;
     int32_t head[kHead];
 #pragma unroll
     for (int j = 0; j < kHead; j++) head[j] = A.in_idx[got && j < len ? rb + j : 0];
     // kFlush bits at a time: their parent-id loads are in flight together
-    // before any store is issued.
+    // before any store is issued.  The values are selected without branches:
+    // a hit's parent and this round's level, a deferred push pair's parent
+    // and the round before's level, -1 for a vertex without in-edges.
     while (bits) {
       int lb[kFlush];
       int32_t pv[kFlush];
+      int16_t lv[kFlush];
+      bool st[kFlush];
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
         lb[k] = bits ? __ffsll((unsigned long long)bits) - 1 : -1;
         bits &= bits - 1;
       }
-      bool hb[kFlush], nb[kFlush], pb[kFlush];
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
         const int lk = lb[k] >= 0 ? lb[k] : 0;
-        hb[k] = lb[k] >= 0 && ((got >> lk) & 1ull);
-        nb[k] = lb[k] >= 0 && ((nmiss >> lk) & 1ull);
-        pb[k] = pend_on && lb[k] >= 0 && ((pl >> lk) & 1ull);
+        const uint64_t m = 1ull << lk;
+        st[k] = lb[k] >= 0 && (wr & m) != 0;
+        const bool hb = (got & m) != 0;
+        const bool pb = pend_on && (pl & m) != 0;
         const int off = pt[lk][lane];
         // no branch: loads go out together; a hit among the row's first
         // kHead entries takes its id from the row head loaded once per chunk,
         // a deferred push pair from its side entry
         const int32_t *src =
-            pb[k] ? reinterpret_cast<const int32_t *>(A.side) + pbase + 2 + __popcll(pl & ((1ull << lk) - 1ull))
-                  : A.in_idx + (hb[k] && !bg && off >= kHead ? rb + off : 0);
+            pb ? reinterpret_cast<const int32_t *>(A.side) + pbase + 2 + __popcll(pl & (m - 1ull))
+               : A.in_idx + (hb && !bg && off >= kHead ? rb + off : 0);
         const int32_t far = *src;
-        pv[k] = pb[k] || off >= kHead ? far : off == 0 ? head[0] : off == 1 ? head[1] : off == 2 ? head[2] : head[3];
+        int32_t p = pb || off >= kHead ? far : off == 0 ? head[0] : off == 1 ? head[1] : off == 2 ? head[2] : head[3];
         if (bgot_any) {  // a big vertex's parent is its candidate
-          const uint32_t cv = A.cand[bg && hb[k] ? (int64_t)bid * 64 + lk : 0];
-          if (bg && hb[k]) pv[k] = (int32_t)cv;
+          const uint32_t cv = A.cand[bg && hb ? (int64_t)bid * 64 + lk : 0];
+          if (bg && hb) p = (int32_t)cv;
         }
+        const int16_t lvl = so.level[lk];
+        pv[k] = hb || pb ? p : (int32_t)-1;
+        lv[k] = hb ? lvl : pb ? (int16_t)(lvl - 1) : (int16_t)-1;
       }
 #pragma unroll
       for (int k = 0; k < kFlush; k++) {
-        if (hb[k] || nb[k] || pb[k]) {
-          st_once(&so.par[lb[k]][v], hb[k] || pb[k] ? pv[k] : (int32_t)-1);
-          st_once(&so.lev[lb[k]][v], hb[k] ? so.level[lb[k]] : pb[k] ? (int16_t)(so.level[lb[k]] - 1) : (int16_t)-1);
+        if (st[k]) {
+          st_once(&so.par[lb[k]][v], pv[k]);
+          st_once(&so.lev[lb[k]][v], lv[k]);
         }
       }
     }
