@@ -1524,9 +1524,13 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
         // would queue there.  A plain read first skips the atomics that
         // cannot change anything.  Within a pass bits are only set and
         // parents only lowered, so a stale read at worst keeps an atomic.
+        // Pass 0 does not read the target's lane word: lanes of f that had
+        // already reached w are set in nx[sw] too, and ms_prep, which reads
+        // w's lane word anyway, clears them (and counts a vertex listed only
+        // for them as nothing new).
         if (phase == 0) {
-          uint64_t nv = f & ~A.vis[w];
-          if (nv && (!kPushFilter || (nx[sw] & nv) != nv)) {
+          const uint64_t nv = f;
+          if (!kPushFilter || (nx[sw] & nv) != nv) {
             uint64_t old = atomicOr((unsigned long long *)&nx[sw], (unsigned long long)nv);
             first = (old & qm) == 0;
           }
@@ -1577,7 +1581,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
   __shared__ unsigned long long tl[192];
   tally_lds_init(tl);
   LaneTally tally;
-  const uint64_t *nx = A.fw[(d + 1) & 1];
+  uint64_t *nx = A.fw[(d + 1) & 1];
   const int32_t *QN = A.q[(d + 1) & 1];
   const int64_t nq = (int64_t)ld_ctrl(&c->qtail[(d + 1) & 1].v);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1597,10 +1601,18 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
     uint64_t nw = 0;
     int64_t od = 0, id = 0;
     int32_t v = 0;
-    uint64_t all = 0;
+    uint64_t all = 0, vw = 0;
     if (i < nq) {
       v = QN[i];
-      all = nx[A.slot_of[v]];
+      const int32_t sv = A.slot_of[v];
+      all = nx[sv];
+      vw = A.vis[v];
+      // pushed lanes that had reached v before (pass 0 sets them unchecked)
+      const uint64_t stale = all & qm & vw;
+      if (stale) {
+        all &= ~stale;
+        nx[sv] = all;
+      }
       nw = all & qm;
     }
     // side entry: the mask's two words and one per new lane, rounded up to
@@ -1632,10 +1644,10 @@ __global__ void __launch_bounds__(kMsBlock) ms_prep(MsArgs A, int tally_on) {
           *gmem(&so.lev[l][v]) = so.level[l];
         }
       }
-      A.vis[v] |= nw;
+      if (nw) A.vis[v] = vw | nw;
       od = A.out_ptr[v + 1] - A.out_ptr[v];
       id = A.symmetric ? od : A.in_ptr[v + 1] - A.in_ptr[v];
-      if ((all & ~qm) == 0) {
+      if (nw && (all & ~qm) == 0) {
         nf++;
         mf += (unsigned long long)od;
       }
