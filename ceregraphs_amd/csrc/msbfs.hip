@@ -1558,8 +1558,9 @@ __global__ void __launch_bounds__(kMsBlock) ms_push(MsArgs A, int phase) {
     }
   }
   if (phase == 0) sq.flush();
+  // the entries are counted once, by the first pass
   unsigned long long s = ms_block_sum(scanned, red);
-  if (threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
+  if (phase == 0 && threadIdx.x == 0 && s) atomicAdd(&c->scanned, s);
   s = ms_block_sum(rows, red);
   if (phase == 0 && threadIdx.x == 0 && s) atomicAdd(&c->rows, s);
 }
