@@ -279,6 +279,11 @@ struct MsGraphData {
   void *seedbuf = nullptr;
   void *seedstage = nullptr;
   size_t seedcap = 0;
+  // Device staging of host outputs (parents, then levels), kept for the next
+  // call with host outputs: a 64-root plane at 2^24 vertices is 6.4 GB, and
+  // allocating it per call cost more than the search.
+  void *outbuf = nullptr;
+  size_t outcap = 0;
   bool stage_busy = false;  // a copy from the staging buffer may be in flight
 };
 
@@ -487,6 +492,7 @@ static void free_msdata(void *q) {
   if (m->exec) hipGraphExecDestroy(m->exec);
   if (m->seedbuf) hipFree(m->seedbuf);
   if (m->seedstage) hipHostFree(m->seedstage);
+  if (m->outbuf) hipFree(m->outbuf);
   if (m->mem) hipFree(m->mem);
   if (m->noin) hipFree(m->noin);
   if (m->out_slot && m->out_slot != m->in_slot) hipFree(m->out_slot);
@@ -1151,11 +1157,10 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
         const uint64_t prev = lane_before(x);
         uint64_t mine = h & ~(lane > head ? prev : 0ull);
         const TileOff off = (TileOff)(base + lane - rs + 1);  // from the row start
-        while (mine) {
-          int l = __ffsll((unsigned long long)mine) - 1;
-          mine &= mine - 1;
-          pt[l][r] = off;
-        }
+        // one 32-bit half at a time: a 64-bit find-first and clear take
+        // twice the instructions
+        for (uint32_t m = (uint32_t)mine; m; m &= m - 1) pt[__builtin_ctz(m)][r] = off;
+        for (uint32_t m = (uint32_t)(mine >> 32); m; m &= m - 1) pt[32 + __builtin_ctz(m)][r] = off;
         // each row owner collects its row's OR over this step
         int last = (my_end < base + kWave ? my_end : base + kWave) - 1 - base;
         bool hit_step = lenr && vstart < base + kWave && my_end > base;
@@ -2192,11 +2197,17 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   const size_t plane = (size_t)g->n * nroots;
   int32_t *dpar = parents;
   int16_t *dlev = levels;
-  void *tmp = nullptr;
   if (!on_device) {
-    CG_HIP(hipMalloc(&tmp, al(plane * sizeof(int32_t)) + plane * sizeof(int16_t) + 256));
-    dpar = (int32_t *)tmp;
-    dlev = (int16_t *)((char *)tmp + al(plane * sizeof(int32_t)));
+    const size_t need = al(plane * sizeof(int32_t)) + plane * sizeof(int16_t) + 256;
+    if (md->outcap < need) {
+      if (md->outbuf) hipFree(md->outbuf);
+      md->outbuf = nullptr;
+      md->outcap = 0;
+      CG_HIP(hipMalloc(&md->outbuf, need));
+      md->outcap = need;
+    }
+    dpar = (int32_t *)md->outbuf;
+    dlev = (int16_t *)((char *)md->outbuf + al(plane * sizeof(int32_t)));
   }
 
   MsArgs A;
@@ -2261,7 +2272,6 @@ static int ms_run(cgbfs_graph *h, const int64_t *sv, const int32_t *sl, int64_t 
   };
   auto cleanup = [&]() {
     for (hipEvent_t e : evs) hipEventDestroy(e);
-    if (tmp) hipFree(tmp);
   };
   hipEvent_t e0, e1;
   hipError_t e = hipMemcpyAsync(md->seedbuf, stage, seed_bytes, hipMemcpyHostToDevice, st);
