@@ -56,7 +56,10 @@ constexpr int kMsWaves = kMsBlock / kWave;
 constexpr int kMsHist = 4096;
 constexpr int64_t kSeg = 256;       // segment length and wavefront-tier cap
 constexpr int kMsLdsQueue = 2048;   // per-block staged queue entries
-constexpr int kUnroll = 2;          // pull steps whose loads are issued together
+#ifndef CG_MS_UNROLL
+#define CG_MS_UNROLL 2
+#endif
+constexpr int kUnroll = CG_MS_UNROLL;  // pull steps whose loads are issued together
 #ifndef CG_MS_FLUSH
 #define CG_MS_FLUSH 4
 #endif
