@@ -520,3 +520,46 @@ def test_workgroup_loop_equal_oracle():
     with pytest.raises(_lib.CgbfsError) as e:
       dg.bfs(0, loop="workgroup")
     assert e.value.code == _lib.CGBFS_EINVAL
+
+
+@pytest.mark.parametrize("loop,capture", [("persistent", False), ("kernels", False),
+                                          ("kernels", True)])
+def test_sources_without_in_edges(loop, capture):
+  """A search starts with every vertex without in-edges marked visited, so
+  no pull level looks at them again: sources that have out-edges but no
+  in-edges (only the source itself is reached among such vertices), a
+  vertex with in-edges only, and an isolated one all give the oracle's
+  results in every level loop and direction."""
+  abcd = (0.45, 0.25, 0.15, 0.15)
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(14, 8, 21, *abcd[:3], directed=True)
+  g = O.Graph(n, ip, ii, op, oi, False)
+  outd, ind = np.diff(op), np.diff(ip)
+  srcs = np.nonzero((outd > 0) & (ind == 0))[0]
+  assert len(srcs) > 3
+  picks = [int(srcs[0]), int(srcs[len(srcs) // 2]), int(srcs[-1]),
+           int(np.argmax(outd * (ind == 0))),
+           int(np.nonzero((outd == 0) & (ind > 0))[0][0]),
+           int(np.nonzero((outd == 0) & (ind == 0))[0][0])]
+  with DeviceGraph.from_rmat(14, 8, 21, abcd=abcd, directed=True) as dg:
+    for s in picks:
+      for d in DIRECTIONS:
+        _check_single(dg, g, s, d, loop=loop, capture=capture)
+
+
+def test_msbfs_host_outputs_reuse_staging():
+  """Host outputs are staged in a device buffer kept with the graph: calls
+  with more, then fewer, then more lanes reuse or grow it, and every lane
+  of every call still equals the oracle."""
+  n, ip, ii, op, oi = gen_rmat.rmat_csr(12, 16, 7)
+  g = O.Graph(n, ip, ii, op, oi, True)
+  deg = np.diff(op)
+  cands = np.nonzero(deg)[0]
+  rng = np.random.default_rng(7)
+  with DeviceGraph.from_rmat(12, 16, 7) as dg:
+    for k in (5, 64, 3, 64):
+      roots = rng.choice(cands, k, replace=False)
+      r = dg.msbfs(roots)
+      for l in (0, k // 2, k - 1):
+        lev, par, _, _ = O.bfs(g, int(roots[l]))
+        np.testing.assert_array_equal(r.levels[l], lev.astype(np.int16), err_msg=f"k={k} lane {l}")
+        np.testing.assert_array_equal(r.parents[l], par, err_msg=f"k={k} lane {l}")
