@@ -62,9 +62,9 @@ def pull_round_bytes(n, rows, edges, pairs, nf, noin_pairs=0):
   words of every vertex (8 V), offsets of the adjacency lists opened (8 U),
   4 B column id + 8 B frontier word per adjacency entry read (12 E), 4 B
   parent + 2 B level per reached (lane, vertex) pair (6 P), and 8 B vis +
-  8 B next word per newly active vertex (16 N).  The first pull round also
-  writes the -1 outputs of every (lane, vertex) pair whose vertex has no
-  in-edges (6 B each, noin_pairs of them)."""
+  8 B next word per newly active vertex (16 N).  The second pull round of a
+  search also writes the -1 outputs of every (lane, vertex) pair whose
+  vertex has no in-edges (6 B each, noin_pairs of them)."""
   return 8 * n + 8 * rows + 12 * edges + 6 * pairs + 16 * nf + 6 * noin_pairs
 
 
@@ -392,8 +392,11 @@ def main():
   # The graph is symmetric, so no in-edges means degree 0; roots have
   # degree >= 1, so every lane misses every such vertex.
   noin_pairs = len(roots) * int(np.count_nonzero(deg == 0))
+  # the second pull round writes the outputs of the vertices without in-edges
+  # (with a single pull round, ms_finalize does, outside the pull)
+  noin_round = pr[1] if len(pr) > 1 else None
   byts = sum(pull_round_bytes(n, st.pull_rows[i], st.pull_edges[i], st.pull_pairs[i],
-                              st.pull_nf[i], noin_pairs if i == pr[0] else 0) for i in pr)
+                              st.pull_nf[i], noin_pairs if i == noin_round else 0) for i in pr)
   tpull = float(st.pull_ms[pr].sum()) / 1e3
   npull = len(pr)
   achieved = byts / tpull / 1e9 if tpull > 0 else 0.0

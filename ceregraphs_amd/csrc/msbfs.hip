@@ -69,6 +69,13 @@ constexpr int kGraphRounds = 8;     // rounds per captured graph (graph replay m
 #define CG_MS_DENSE_GRAB 1
 #endif
 constexpr int32_t kGrabDense = CG_MS_DENSE_GRAB;  // chunks per grab of a dense pull
+// Which pull round of a search writes the outputs of the vertices without
+// in-edges (-1 in every lane but their own seeds'), in the same store
+// instructions as its own new pairs.
+#ifndef CG_MS_NOIN_PULL
+#define CG_MS_NOIN_PULL 2
+#endif
+constexpr int kNoinPull = CG_MS_NOIN_PULL;
 // Waves per SIMD the chunk pull is compiled for (its register budget is
 // 512 / kMsPullWaves VGPRs).
 #ifndef CG_MS_PULL_WAVES
@@ -121,7 +128,9 @@ struct MsCtrl {
   int done;
   int status;
   int grab;                    // chunks per grab of that round's pull; < 0: a static split
-  int noin_left;               // outputs of vertices without in-edges not yet written
+  // pull rounds until the one that writes the outputs of the vertices
+  // without in-edges (1: the next pull round); 0 once they are written
+  int noin_left;
   // Entries of the previous round's new-vertex list (q[rounds & 1]) whose
   // push pairs wait in the side table for their outputs (0: none).
   long long pend_n;
@@ -842,7 +851,7 @@ __device__ __forceinline__ RoundState round_state(MsCtrl *c) {
   r.listed = ld_ctrl(&c->listed) != 0;
   r.use_fb = ld_ctrl(&c->use_fb) != 0;
   r.grab = ld_ctrl(&c->grab);
-  r.noin = ld_ctrl(&c->noin_left) != 0;
+  r.noin = ld_ctrl(&c->noin_left) == 1;
   r.pend_n = (int64_t)ld_ctrl(&c->pend_n);
   return r;
 }
@@ -916,7 +925,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     st_ctrl(&c->done, 0);
     st_ctrl(&c->status, 0);
     st_ctrl(&c->grab, P.dense_grab);
-    st_ctrl(&c->noin_left, 1);
+    st_ctrl(&c->noin_left, kNoinPull);
     st_ctrl(&c->pend_n, 0ll);
     reset_ms_cursors(A.cursor);
   }
@@ -1173,9 +1182,10 @@ __global__ void __launch_bounds__(kMsBlock, kMsPullWaves) ms_pull(MsArgs A) {
       if (!__any(lenr && (need & ~got))) break;  // every row of the chunk satisfied
     }
     // Vertices without in-edges: no lane but a seed's own ever reaches them.
-    // The first pull round writes their -1 outputs (lanes other than their
+    // The second pull round writes their -1 outputs (lanes other than their
     // own seeds') in the same store instructions as the chunk's new pairs,
-    // so those partial lines fill up instead of being written twice.
+    // so those partial lines fill up instead of being written twice; the
+    // first pull round, the heaviest, is left its own stores.
     const uint64_t nmiss = (R.noin && ((A.noin[ch] >> lane) & 1ull)) ? A.act & ~vw : 0ull;
     const bool bg = (bigm >> lane) & 1ull;  // this lane's vertex is big
     if (bg) got = A.bgot[bid];
@@ -1832,7 +1842,11 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   st_ctrl(&c->segtail[1].v, 0ull);
   st_ctrl(&c->segtail_late, 0ull);
   reset_ms_cursors(A.cursor);
-  if (cur_pm) st_ctrl(&c->noin_left, 0);  // this round's pull visited every vertex
+  // a pull visits every vertex: the one the count reaches wrote them
+  if (cur_pm) {
+    const int left = ld_ctrl(&c->noin_left);
+    if (left > 0) st_ctrl(&c->noin_left, left - 1);
+  }
   // A pull rescans every row it opens from the start, so the next pull reads
   // at most what this one read: below one entry per vertex it is light, and
   // its chunks, most of which have nothing to scan, are split statically.
@@ -1931,7 +1945,7 @@ __global__ void ms_seed_outputs(MsArgs A, const int64_t *sv, const int32_t *sl, 
 __global__ void __launch_bounds__(kMsBlock) ms_finalize(MsArgs A) {
   const int lane = lane_id();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // vertices without in-edges were written by the first pull round, if any
+  // vertices without in-edges were written by a pull round, if one came
   const bool noin_done = ld_ctrl(&A.ctrl->noin_left) == 0;
   for (int64_t v0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane; v0 < A.n; v0 += stride) {
     const int64_t v = v0 + lane;
