@@ -1250,7 +1250,7 @@ void fill_options(const cgbfs_options *opt, int64_t n, bool multi_source, cgbfs_
   // 0.57, s23 0.81 / 0.67 / 0.73, s24 1.45 / 1.07 / 1.04.
   if (o->grid_mult <= 0) {
     if (multi_source) o->grid_mult = n < (1 << 19) ? 2 : n < (1 << 23) ? 4 : 8;
-    else o->grid_mult = n <= (1 << 21) ? 2 : n <= (1 << 23) ? 4 : 8;
+    else o->grid_mult = n <= (1 << 21) ? 2 : n <= (1 << 24) ? 4 : 8;
   }
   if (o->seg_grab <= 0) o->seg_grab = 32;
   if (o->push_wave_degree <= 0) o->push_wave_degree = (int32_t)kWaveDegree;

@@ -77,7 +77,7 @@ typedef struct {
                          dense pull round, 1..1024 (8 single-root,
                          1 multi-source)                                    */
   int32_t grid_mult;  /* blocks per CU for grid-stride kernels, 1..64
-                         (single root: 2 up to 2^21 vertices, 4 up to 2^23,
+                         (single root: 2 up to 2^21 vertices, 4 up to 2^24,
                          else 8, multi-source: 2 below 2^19, 4 below 2^23,
                          else 8)                                            */
   int64_t max_levels; /* abort with CGBFS_ETIMEOUT beyond this        (n+1)  */
