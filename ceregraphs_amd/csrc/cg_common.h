@@ -227,6 +227,7 @@ struct ChunkFeed {
       return true;
     }
     for (;;) {
+      if (tried >= kParts) return false;  // drained (asked again after false)
       if (!pending) issue();
       pending = false;
       const int p = part();
