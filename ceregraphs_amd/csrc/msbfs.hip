@@ -781,6 +781,15 @@ __device__ __forceinline__ uint64_t seg_step(const MsArgs &A, int64_t beg, int k
 // among its first, lowest-id in-neighbours (hubs themselves).
 constexpr int kSegHeadLanes = 8;
 
+// A pull round reads the frontier bits (one per slot, 2 MB at 2^24
+// vertices, so they stay in L2) before it gathers a lane word, and skips
+// the words that are zero, when the frontier holds at most n / kFbDenom
+// vertices; above that the words are gathered directly.
+#ifndef CG_FB_DENOM
+#define CG_FB_DENOM 8
+#endif
+constexpr long long kFbDenom = CG_FB_DENOM;
+
 __device__ __forceinline__ uint64_t ms_seg_pull(const MsArgs &A, const uint64_t *fr, int64_t beg,
                                                 int64_t end, uint64_t need, uint32_t *cand,
                                                 bool use_fb, uint32_t &scanned) {
@@ -970,7 +979,7 @@ __global__ void ms_init_seeds(MsArgs A, const int64_t *sv, const int32_t *sl, in
     const long long nq0 = (long long)ld_ctrl(&c->qtail[0].v);
     st_ctrl(&c->prev_nf, nq0);
     st_ctrl(&c->listed, 1);
-    st_ctrl(&c->use_fb, nq0 * 8 <= (long long)A.n ? 1 : 0);
+    st_ctrl(&c->use_fb, nq0 * kFbDenom <= (long long)A.n ? 1 : 0);
     st_ctrl(&c->pmask, (unsigned long long)pm);
     st_ctrl(&c->qmask, (unsigned long long)qm);
   }
@@ -1863,7 +1872,7 @@ __global__ void ms_decide(MsArgs A, MsParams P) {
   st_ctrl(&c->prev_nf, nf);
   st_ctrl(&c->gdir, g);
   st_ctrl(&c->listed, listed_next ? 1 : 0);
-  st_ctrl(&c->use_fb, nf * 8 <= (long long)A.n ? 1 : 0);  // frontier bits pay off below n/8
+  st_ctrl(&c->use_fb, nf * kFbDenom <= (long long)A.n ? 1 : 0);
   bool done = nf == 0;
   if (!done && d + 1 >= P.max_levels) {
     st_ctrl(&c->status, (int)CGBFS_ETIMEOUT);
